@@ -1,0 +1,80 @@
+"""Configuration: a module that behaves as a dictionary of options.
+
+Lookup order (reference: pyop2/configuration.py:54-106): keyword arguments to
+:func:`configure`, then the file named by ``config=``, else ``./pyop2.yaml``
+when present, then the defaults below.  Options are read as ``cfg['key']``
+or ``cfg.key``.
+"""
+
+import sys
+import types
+import warnings
+
+import yaml
+
+DEFAULTS = {
+    'log_level': 'WARN',
+    'python_plan': True,
+    'backend': 'hip',
+    'debug': 0,
+    'dump-gencode': False,
+    'dump-gencode-path': '/tmp/%(kernel)s-%(time)s.hip',
+    # hip backend options
+    'hip_partition_size': 0,      # 0: taken from the autotuner table
+    'hip_block_size': 0,          # 0: taken from the autotuner table
+    'hip_reorder': True,          # locality-ordered partitions for indirect loops
+    'hip_autotune': False,
+    'hip_cache_dir': None,        # on-disk code-object cache (None: ~/.cache/pyop2_amd)
+}
+
+
+class ConfigModule(types.ModuleType):
+    """Module object answering ``cfg[key]`` and ``cfg.key``."""
+
+    _config = None
+
+    def configure(self, **kwargs):
+        entries = dict(DEFAULTS)
+        cfgfile = kwargs.pop('config', None)
+        if cfgfile is not None:
+            try:
+                src = cfgfile.read() if hasattr(cfgfile, 'read') else open(cfgfile).read()
+                entries.update(yaml.safe_load(src) or {})
+            except IOError:
+                pass
+        else:
+            try:
+                with open('pyop2.yaml') as f:
+                    entries.update(yaml.safe_load(f) or {})
+            except IOError:
+                pass
+        entries.update(kwargs)
+        self._config = entries
+        if entries.get('debug', 0) > 0:
+            warnings.simplefilter("always")
+
+    def reset(self):
+        """Forget all configuration entries."""
+        self._config = None
+
+    def get(self, key, default=None):
+        """Return an option, falling back to the defaults before configure()."""
+        if self._config is None:
+            return DEFAULTS.get(key, default)
+        return self._config.get(key, DEFAULTS.get(key, default))
+
+    def __getitem__(self, key):
+        if not self._config:
+            raise KeyError(key)
+        return self._config[key]
+
+    def __getattr__(self, name):
+        if name.startswith('__') or not self._config:
+            raise AttributeError(name)
+        try:
+            return self._config[name]
+        except KeyError:
+            raise AttributeError(name)
+
+
+sys.modules[__name__].__class__ = ConfigModule

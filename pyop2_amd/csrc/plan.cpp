@@ -1,0 +1,383 @@
+// Execution plan for indirect loops with conflicting writes.
+//
+// Replaces the reference's Cython plan (pyop2/plan.pyx:68-456).  The
+// reference cuts the iteration set into contiguous blocks of `part_size`
+// elements (plan.pyx:107-110) and colours blocks, then threads within a
+// block, with greedy 32-colour bit-mask rounds (plan.pyx:214-376).  On meshes
+// whose numbering is permuted, contiguous blocks touch targets all over the
+// mesh and need as many colours as there are blocks, so this plan first
+// orders the iteration set into compact regions grown breadth-first through
+// the conflict maps, then colours with 64-bit masks:
+//
+//   elems[n]            execution order (blocks are consecutive runs of it)
+//   offset/nelem[b]     the run of block b inside elems
+//   thrcol[n]           thread colour of elems[i]; nthrcol[b] colours in b
+//   blkcol[b]           block colour; blocks of one colour share no target
+//   blkmap[nblocks]     block ids sorted (stably) by colour; ncolblk[c]
+//
+// Invariants (checked by op2hip_plan_verify): no two elements of one
+// (block, thread colour) share a race target, and no two blocks of one
+// colour share a race target.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+struct RaceMap {
+  const int* values;
+  int dim;
+  int idx;    // -1: every column of the map
+  int space;  // id of the target index space (a Dat, or a Mat's rows)
+};
+
+struct Plan {
+  int nelems = 0;
+  int nblocks = 0;
+  int ncolors = 0;
+  int max_nelem = 0;
+  int max_thrcol = 0;
+  std::vector<int> elems, offset, nelem, thrcol, nthrcol, blkcol, blkmap, ncolblk;
+};
+
+template <class F>
+inline void for_targets(const std::vector<RaceMap>& maps, int e, F&& f) {
+  for (size_t m = 0; m < maps.size(); ++m) {
+    const RaceMap& rm = maps[m];
+    const int* row = rm.values + static_cast<int64_t>(e) * rm.dim;
+    if (rm.idx >= 0) {
+      f(rm.space, row[rm.idx]);
+    } else {
+      for (int k = 0; k < rm.dim; ++k) f(rm.space, row[k]);
+    }
+  }
+}
+
+// Greedy colouring of `items` (each a list of (space, target) touches) with
+// 64-bit masks per target, in rounds of 64 colours.  `touch(item, cb)` calls
+// cb(space, target) for every target of an item.  Returns colours per item.
+template <class Touch>
+int greedy_colour(int nitems, std::vector<std::vector<uint64_t>>& masks, Touch&& touch,
+                  std::vector<int>& colour) {
+  colour.assign(nitems, -1);
+  int base = 0;
+  int remaining = nitems;
+  int ncolours = 0;
+  while (remaining > 0) {
+    std::vector<std::pair<int, int>> touched;  // (space, target) to reset
+    for (int it = 0; it < nitems; ++it) {
+      if (colour[it] >= 0) continue;
+      uint64_t mask = 0;
+      touch(it, [&](int s, int t) { mask |= masks[s][t]; });
+      if (mask == ~0ull) continue;  // no free colour in this round
+      int c = __builtin_ctzll(~mask);
+      colour[it] = base + c;
+      ncolours = std::max(ncolours, base + c + 1);
+      --remaining;
+      touch(it, [&](int s, int t) {
+        if (masks[s][t] == 0) touched.emplace_back(s, t);
+        masks[s][t] |= (1ull << c);
+      });
+    }
+    for (auto& st : touched) masks[st.first][st.second] = 0;
+    base += 64;
+  }
+  return ncolours;
+}
+
+// Breadth-first region growing over the element graph (elements adjacent
+// when they share a target of a race map).  Regions hold `ps` elements; the
+// seed of the next region is the oldest unassigned element on the frontier
+// of earlier regions, so regions tile the mesh in a sweep.
+void grow_regions(int n, int ps, const std::vector<RaceMap>& maps,
+                  const std::vector<int>& space_size, std::vector<int>& order,
+                  std::vector<int>& region_start) {
+  // target -> element incidence (CSR) per space
+  size_t nspace = space_size.size();
+  std::vector<std::vector<int>> ptr(nspace), adj(nspace);
+  for (size_t s = 0; s < nspace; ++s) ptr[s].assign(space_size[s] + 1, 0);
+  for (int e = 0; e < n; ++e)
+    for_targets(maps, e, [&](int s, int t) { ptr[s][t + 1]++; });
+  for (size_t s = 0; s < nspace; ++s) {
+    for (int t = 0; t < space_size[s]; ++t) ptr[s][t + 1] += ptr[s][t];
+    adj[s].resize(ptr[s][space_size[s]]);
+  }
+  {
+    std::vector<std::vector<int>> cur(nspace);
+    for (size_t s = 0; s < nspace; ++s) cur[s].assign(ptr[s].begin(), ptr[s].end() - 1);
+    for (int e = 0; e < n; ++e)
+      for_targets(maps, e, [&](int s, int t) { adj[s][cur[s][t]++] = e; });
+  }
+  std::vector<int> region(n, -1);
+  std::deque<int> frontier;
+  std::vector<int> q;
+  q.reserve(ps);
+  order.clear();
+  order.reserve(n);
+  region_start.clear();
+  int next_scan = 0;
+  int r = 0;
+  while (static_cast<int>(order.size()) < n) {
+    int seed = -1;
+    while (!frontier.empty()) {
+      int x = frontier.front();
+      frontier.pop_front();
+      if (region[x] < 0) {
+        seed = x;
+        break;
+      }
+    }
+    if (seed < 0) {
+      while (region[next_scan] >= 0) ++next_scan;
+      seed = next_scan;
+    }
+    region_start.push_back(static_cast<int>(order.size()));
+    q.clear();
+    q.push_back(seed);
+    region[seed] = r;
+    size_t head = 0;
+    while (head < q.size()) {
+      int x = q[head++];
+      for_targets(maps, x, [&](int s, int t) {
+        for (int k = ptr[s][t]; k < ptr[s][t + 1]; ++k) {
+          int y = adj[s][k];
+          if (region[y] >= 0) continue;
+          if (static_cast<int>(q.size()) < ps) {
+            region[y] = r;
+            q.push_back(y);
+          } else {
+            frontier.push_back(y);
+          }
+        }
+      });
+    }
+    std::sort(q.begin(), q.end());
+    order.insert(order.end(), q.begin(), q.end());
+    ++r;
+  }
+  region_start.push_back(n);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Build a plan over the iteration range [0, nelems).
+//   maps[i]       host int32 map values, row-major (rows >= nelems, dims[i] cols)
+//   idxs[i]       column used (-1: all columns)
+//   spaces[i]     target index space of map i (0..nspaces-1)
+//   space_size[s] number of targets in space s
+//   reorder       1: locality-ordered regions, 0: contiguous blocks
+int op2hip_plan_build(int nelems, int part_size, int reorder, int nmaps, const int* const* maps,
+                      const int* dims, const int* idxs, const int* spaces, int nspaces,
+                      const int* space_size, void** plan_out) {
+  OP2HIP_REQUIRE(plan_out, "null plan_out");
+  OP2HIP_REQUIRE(nelems >= 0 && part_size > 0, "bad plan sizes");
+  OP2HIP_REQUIRE(nmaps >= 0 && nspaces >= 0, "bad map counts");
+  std::vector<RaceMap> rmaps;
+  for (int i = 0; i < nmaps; ++i) {
+    OP2HIP_REQUIRE(maps[i] && dims[i] > 0, "bad race map %d", i);
+    OP2HIP_REQUIRE(spaces[i] >= 0 && spaces[i] < nspaces, "bad space of map %d", i);
+    OP2HIP_REQUIRE(idxs[i] < dims[i], "bad index of map %d", i);
+    rmaps.push_back({maps[i], dims[i], idxs[i], spaces[i]});
+  }
+  std::vector<int> ssize(space_size, space_size + nspaces);
+  // every target must lie inside its space
+  for (int e = 0; e < nelems; ++e) {
+    bool ok = true;
+    int bad_s = 0, bad_t = 0;
+    for_targets(rmaps, e, [&](int s, int t) {
+      if (t < 0 || t >= ssize[s]) {
+        ok = false;
+        bad_s = s;
+        bad_t = t;
+      }
+    });
+    OP2HIP_REQUIRE(ok, "map value %d out of range [0,%d) at element %d", bad_t, ssize[bad_s], e);
+  }
+  Plan* p = new Plan();
+  p->nelems = nelems;
+  std::vector<int> starts;
+  if (reorder && !rmaps.empty() && nelems > 0) {
+    grow_regions(nelems, part_size, rmaps, ssize, p->elems, starts);
+  } else {
+    p->elems.resize(nelems);
+    for (int e = 0; e < nelems; ++e) p->elems[e] = e;
+    for (int s = 0; s < nelems; s += part_size) starts.push_back(s);
+    starts.push_back(nelems);
+  }
+  p->nblocks = static_cast<int>(starts.size()) - 1;
+  p->offset.resize(p->nblocks);
+  p->nelem.resize(p->nblocks);
+  for (int b = 0; b < p->nblocks; ++b) {
+    p->offset[b] = starts[b];
+    p->nelem[b] = starts[b + 1] - starts[b];
+    p->max_nelem = std::max(p->max_nelem, p->nelem[b]);
+  }
+  std::vector<std::vector<uint64_t>> masks(nspaces);
+  for (int s = 0; s < nspaces; ++s) masks[s].assign(ssize[s], 0);
+  // thread colours, block by block
+  p->thrcol.assign(nelems, 0);
+  p->nthrcol.assign(p->nblocks, rmaps.empty() ? 1 : 0);
+  if (!rmaps.empty()) {
+    std::vector<int> col;
+    for (int b = 0; b < p->nblocks; ++b) {
+      const int* el = p->elems.data() + p->offset[b];
+      int nb = p->nelem[b];
+      int nc = greedy_colour(
+          nb, masks,
+          [&](int it, auto&& cb) { for_targets(rmaps, el[it], cb); }, col);
+      for (int k = 0; k < nb; ++k) p->thrcol[p->offset[b] + k] = col[k];
+      p->nthrcol[b] = nc;
+      p->max_thrcol = std::max(p->max_thrcol, nc);
+    }
+  } else {
+    p->max_thrcol = 1;
+  }
+  // block colours
+  std::vector<int> bcol;
+  if (!rmaps.empty()) {
+    p->ncolors = greedy_colour(
+        p->nblocks, masks,
+        [&](int b, auto&& cb) {
+          const int* el = p->elems.data() + p->offset[b];
+          for (int k = 0; k < p->nelem[b]; ++k) for_targets(rmaps, el[k], cb);
+        },
+        bcol);
+  } else {
+    bcol.assign(p->nblocks, 0);
+    p->ncolors = p->nblocks > 0 ? 1 : 0;
+  }
+  p->blkcol = bcol;
+  p->ncolblk.assign(p->ncolors, 0);
+  for (int b = 0; b < p->nblocks; ++b) p->ncolblk[bcol[b]]++;
+  p->blkmap.resize(p->nblocks);
+  std::vector<int> cur(p->ncolors + 1, 0);
+  for (int c = 0; c < p->ncolors; ++c) cur[c + 1] = cur[c] + p->ncolblk[c];
+  for (int b = 0; b < p->nblocks; ++b) p->blkmap[cur[bcol[b]]++] = b;
+  *plan_out = p;
+  return OP2HIP_OK;
+}
+
+enum {
+  PLAN_NELEMS = 0,
+  PLAN_NBLOCKS = 1,
+  PLAN_NCOLORS = 2,
+  PLAN_MAX_NELEM = 3,
+  PLAN_MAX_THRCOL = 4,
+};
+
+int op2hip_plan_scalar(void* plan, int field, int64_t* out) {
+  Plan* p = static_cast<Plan*>(plan);
+  OP2HIP_REQUIRE(p && out, "null plan");
+  switch (field) {
+    case PLAN_NELEMS: *out = p->nelems; break;
+    case PLAN_NBLOCKS: *out = p->nblocks; break;
+    case PLAN_NCOLORS: *out = p->ncolors; break;
+    case PLAN_MAX_NELEM: *out = p->max_nelem; break;
+    case PLAN_MAX_THRCOL: *out = p->max_thrcol; break;
+    default: OP2HIP_REQUIRE(false, "unknown plan field %d", field);
+  }
+  return OP2HIP_OK;
+}
+
+// Copy one plan array into `dst` (host int32 of the array's length):
+// 0 elems, 1 offset, 2 nelem, 3 thrcol, 4 nthrcol, 5 blkcol, 6 blkmap, 7 ncolblk.
+int op2hip_plan_array(void* plan, int which, int* dst) {
+  Plan* p = static_cast<Plan*>(plan);
+  OP2HIP_REQUIRE(p && dst, "null plan");
+  const std::vector<int>* v = nullptr;
+  switch (which) {
+    case 0: v = &p->elems; break;
+    case 1: v = &p->offset; break;
+    case 2: v = &p->nelem; break;
+    case 3: v = &p->thrcol; break;
+    case 4: v = &p->nthrcol; break;
+    case 5: v = &p->blkcol; break;
+    case 6: v = &p->blkmap; break;
+    case 7: v = &p->ncolblk; break;
+    default: OP2HIP_REQUIRE(false, "unknown plan array %d", which);
+  }
+  if (!v->empty()) memcpy(dst, v->data(), v->size() * sizeof(int));
+  return OP2HIP_OK;
+}
+
+int op2hip_plan_destroy(void* plan) {
+  delete static_cast<Plan*>(plan);
+  return OP2HIP_OK;
+}
+
+// Check the conflict-freedom invariants of a plan (given as arrays, so a
+// plan from any source can be checked) against the race maps.  Sets *ok to
+// 1 when every element appears once, no (block, thread colour) touches a
+// target from two elements, and no two blocks of one colour share a target.
+// One element naming a target twice is one kernel call, not a conflict.
+int op2hip_plan_verify(int nelems, int nblocks, const int* elems, const int* offset,
+                       const int* nelem, const int* thrcol, const int* blkcol, int nmaps,
+                       const int* const* maps, const int* dims, const int* idxs,
+                       const int* spaces, int nspaces, const int* space_size, int* ok) {
+  OP2HIP_REQUIRE(ok, "null ok");
+  std::vector<RaceMap> rmaps;
+  for (int i = 0; i < nmaps; ++i) rmaps.push_back({maps[i], dims[i], idxs[i], spaces[i]});
+  *ok = 1;
+  std::vector<char> seen(nelems, 0);
+  int64_t total = 0;
+  for (int b = 0; b < nblocks; ++b) {
+    for (int k = 0; k < nelem[b]; ++k) {
+      int e = elems[offset[b] + k];
+      if (e < 0 || e >= nelems || seen[e]) {
+        *ok = 0;
+        return OP2HIP_OK;
+      }
+      seen[e] = 1;
+      ++total;
+    }
+  }
+  if (total != nelems) {
+    *ok = 0;
+    return OP2HIP_OK;
+  }
+  // stamp[s][t]: last (block, colour) tag; bstamp[s][t]: last block;
+  // bcolour[s][t]: colours of the blocks that touched t, as a set of
+  // (colour, block) pairs kept in a flat per-target list
+  std::vector<std::vector<int64_t>> stamp(nspaces);
+  for (int s = 0; s < nspaces; ++s) stamp[s].assign(space_size[s], -1);
+  std::vector<std::pair<int, int>> ts;
+  for (int b = 0; b < nblocks && *ok; ++b) {
+    for (int k = 0; k < nelem[b] && *ok; ++k) {
+      int pos = offset[b] + k;
+      int64_t tag = static_cast<int64_t>(b) * 1000003 + thrcol[pos];
+      ts.clear();
+      for_targets(rmaps, elems[pos], [&](int s, int t) { ts.emplace_back(s, t); });
+      std::sort(ts.begin(), ts.end());
+      ts.erase(std::unique(ts.begin(), ts.end()), ts.end());
+      for (auto& st : ts) {
+        if (stamp[st.first][st.second] == tag) *ok = 0;
+        stamp[st.first][st.second] = tag;
+      }
+    }
+  }
+  if (!*ok) return OP2HIP_OK;
+  std::vector<std::vector<std::vector<std::pair<int, int>>>> touches(nspaces);
+  for (int s = 0; s < nspaces; ++s) touches[s].resize(space_size[s]);
+  for (int b = 0; b < nblocks && *ok; ++b) {
+    for (int k = 0; k < nelem[b]; ++k) {
+      for_targets(rmaps, elems[offset[b] + k], [&](int s, int t) {
+        auto& lst = touches[s][t];
+        bool have = false;
+        for (auto& cb : lst) {
+          if (cb.second == b) have = true;
+          else if (cb.first == blkcol[b]) *ok = 0;
+        }
+        if (!have) lst.emplace_back(blkcol[b], b);
+      });
+    }
+  }
+  return OP2HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,636 @@
+"""The ``hip`` backend: op2 on one MI355X (gfx950) GPU.
+
+Selected with ``op2.init(backend='hip')`` through the plugin boundary of
+:mod:`pyop2_amd.backends` (reference: pyop2/backends.py:128-146).  It exports
+the classes the boundary looks up by name (Set, Map, Dat, Global, Const,
+Kernel, Sparsity, Mat, Solver, IterationSpace, Halo, Arg, ParLoop), plus
+``par_loop``, ``solve``, ``_setup`` and ``MPI``.
+
+Data live in device memory owned by torch (used only as an allocator and for
+the current stream); every computation is a kernel of ``libop2hip.so`` or a
+hiprtc-compiled loop:
+
+* Dat/Global values have a host copy (numpy) and a device copy with the
+  coherence states of the reference's device backends (pyop2/device.py:103-183):
+  HOST (host newer), DEVICE (device newer) or BOTH.  ``.data`` brings values
+  to the host, a ``par_loop`` brings its arguments to the device.
+* Maps are uploaded once as int32.
+* Sparsities are built on the device (bit-exact CSR), Mats hold FP64 CSR
+  values on the device, ``solve`` runs the device Jacobi-PCG.
+* Loops run the stream or coloured templates of :mod:`pyop2_amd.codegen`;
+  coloured loops use a cached :class:`Plan`.
+"""
+
+import ctypes
+import weakref
+
+import numpy as np
+
+from . import base
+from . import configuration as cfg
+from . import _native as nat
+from . import jit
+from .base import (READ, WRITE, RW, INC, MIN, MAX, IdentityMap, IterationIndex, i,
+                   IterationSpace, Halo, Access)
+from .codegen import LoopSource
+from .exceptions import MatTypeError, DatTypeError
+from .logger import debug, warning
+from .mpi import MPI
+from .utils import as_tuple, maybe_setflags, validate_type
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the runtime
+    torch = None
+
+HOST, DEVICE, BOTH = 'HOST', 'DEVICE', 'BOTH'
+
+_state = {'initialised': False, 'device': None}
+
+
+def _setup():
+    """Select GPU 0 and initialise the native runtime (called by op2.init)."""
+    if torch is None or not torch.cuda.is_available():
+        raise RuntimeError("the hip backend needs a ROCm GPU (torch.cuda is unavailable)")
+    nat.check(nat.lib.op2hip_init(0), 'op2hip_init')
+    _state['device'] = torch.device('cuda', 0)
+    torch.cuda.set_device(0)
+    _state['initialised'] = True
+
+
+def device():
+    if not _state['initialised']:
+        _setup()
+    return _state['device']
+
+
+def stream():
+    """The HIP stream every loop is launched on: torch's current stream."""
+    return torch.cuda.current_stream(device()).cuda_stream
+
+
+def _dev_bytes(nbytes):
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device())
+
+
+def _ptr(t):
+    return t.data_ptr() if t is not None else 0
+
+
+def _h2d(dev, host):
+    """Copy a host numpy array's bytes into a device byte tensor."""
+    if host.nbytes == 0:
+        return
+    src = torch.from_numpy(np.ascontiguousarray(host).reshape(-1).view(np.uint8))
+    dev[:host.nbytes].copy_(src)
+
+
+def _d2h(host, dev):
+    """Copy a device byte tensor into a host numpy array (blocking)."""
+    if host.nbytes == 0:
+        return
+    flags = host.flags.writeable
+    if not flags and host.flags['OWNDATA']:
+        host.setflags(write=True)
+    dst = torch.from_numpy(host.reshape(-1).view(np.uint8))
+    dst.copy_(dev[:host.nbytes])
+    if not flags and host.flags['OWNDATA']:
+        host.setflags(write=False)
+
+
+class _DeviceData(object):
+    """Host/device coherence of one array (Dat, Global)."""
+
+    def _dev_init(self):
+        self._dev = None
+        self._state = HOST
+
+    def _to_device(self):
+        if self._dev is None:
+            self._dev = _dev_bytes(self._data.nbytes)
+            _h2d(self._dev, self._data)
+            self._state = BOTH
+        elif self._state == HOST:
+            _h2d(self._dev, self._data)
+            self._state = BOTH
+        return self._dev
+
+    def _sync_to_host(self):
+        if getattr(self, '_state', HOST) == DEVICE:
+            _d2h(self._data, self._dev)
+            self._state = BOTH
+
+    def _host_written(self):
+        self._state = HOST
+
+    def _device_written(self):
+        self._state = DEVICE
+
+
+class Arg(base.Arg):
+    pass
+
+
+class Set(base.Set):
+    pass
+
+
+class Map(base.Map):
+
+    def _device_values(self):
+        d = getattr(self, '_dev', None)
+        if d is None:
+            v = np.ascontiguousarray(self._values, dtype=np.int32)
+            d = _dev_bytes(v.nbytes)
+            _h2d(d, v)
+            self._dev = d
+        return d
+
+
+class Dat(_DeviceData, base.Dat):
+
+    def __init__(self, *args, **kwargs):
+        base.Dat.__init__(self, *args, **kwargs)
+        self._dev_init()
+
+
+class Global(_DeviceData, base.Global):
+
+    def __init__(self, *args, **kwargs):
+        base.Global.__init__(self, *args, **kwargs)
+        self._dev_init()
+
+
+class Const(base.Const):
+    pass
+
+
+class Kernel(base.Kernel):
+    _cache = {}
+
+
+class Sparsity(base.Sparsity):
+    """CSR structure built on the device (bit-exact to the reference's
+    sparsity_utils.cxx:5-51 std::set construction)."""
+
+    _cache = {}
+
+    def _build(self):
+        rmult, cmult = self._dims
+        nrows = self._nrows
+        lsize = nrows * rmult
+        s = stream()
+        rmaps = [m._device_values() for m in self._rmaps]
+        cmaps = [m._device_values() for m in self._cmaps]
+        for m in self._rmaps + self._cmaps:
+            v = m.values
+            if v.size and (v.min() < 0 or v.max() >= m.dataset.total_size):
+                raise base.MapValueError("Map %s has values outside its dataset" % m.name)
+        rowptr = torch.empty(lsize + 1, dtype=torch.int32, device=device())
+        handle = ctypes.c_void_p()
+        nnz = ctypes.c_longlong()
+        nel = (ctypes.c_longlong * self._nmaps)(*[m.iterset.size for m in self._rmaps])
+        nat.check(nat.lib.op2hip_sparsity_build(
+            rmult, cmult, nrows, self._nmaps, nat.ptr_array([_ptr(t) for t in rmaps]),
+            nat.int_array([m.dim for m in self._rmaps]),
+            nat.ptr_array([_ptr(t) for t in cmaps]),
+            nat.int_array([m.dim for m in self._cmaps]), nel, ctypes.c_void_p(_ptr(rowptr)),
+            ctypes.byref(handle), ctypes.byref(nnz), ctypes.c_void_p(s)), 'sparsity build')
+        colidx = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=device())
+        nnz_rows = torch.empty(max(lsize, 1), dtype=torch.int32, device=device())
+        nat.check(nat.lib.op2hip_sparsity_fill(handle, ctypes.c_void_p(_ptr(rowptr)),
+                                               ctypes.c_void_p(_ptr(colidx)),
+                                               ctypes.c_void_p(_ptr(nnz_rows)),
+                                               ctypes.c_void_p(s)), 'sparsity fill')
+        self._dev_rowptr = rowptr
+        self._dev_colidx = colidx
+        self._rowptr = rowptr.cpu().numpy()
+        self._colidx = colidx[:nnz.value].cpu().numpy()
+        self._d_nnz = nnz_rows[:lsize].cpu().numpy()
+        self._d_nz = int(nnz.value)
+        self._o_nnz = np.zeros(lsize, dtype=np.int32)
+        self._o_nz = 0
+
+
+class Mat(base.Mat):
+    """FP64 CSR values on the device, aligned with the Sparsity's CSR
+    (reference: petsc_base.py:80-120 createAIJWithArrays)."""
+
+    def __init__(self, *args, **kwargs):
+        base.Mat.__init__(self, *args, **kwargs)
+        if self._datatype != np.float64:
+            raise RuntimeError("Can only create a matrix of type float64, %s is not supported"
+                               % self._datatype)
+        nnz = self._sparsity.nz
+        self._dev_vals = torch.zeros(max(nnz, 1), dtype=torch.float64, device=device())
+        self._err = torch.zeros(1, dtype=torch.int32, device=device())
+
+    def _check_err(self):
+        if int(self._err.item()) != 0:
+            self._err.zero_()
+            raise RuntimeError("Mat %s: an element added an entry outside its sparsity pattern"
+                               % self.name)
+
+    def zero(self):
+        """Zero every stored value."""
+        self._dev_vals.zero_()
+
+    def zero_rows(self, rows, diag_val):
+        """Zero the given rows except the diagonal, which becomes ``diag_val``
+        (reference: petsc_base.py:131-135)."""
+        r = np.asarray(rows, dtype=np.int32).reshape(-1)
+        if r.size == 0:
+            return
+        nrows = self._sparsity.nrows * self.dims[0]
+        if r.min() < 0 or r.max() >= nrows:
+            raise IndexError("row out of range in zero_rows")
+        d = torch.from_numpy(r).to(device())
+        sp = self._sparsity
+        nat.check(nat.lib.op2hip_csr_zero_rows(int(r.size), ctypes.c_void_p(_ptr(d)),
+                                               ctypes.c_void_p(_ptr(sp._dev_rowptr)),
+                                               ctypes.c_void_p(_ptr(sp._dev_colidx)),
+                                               ctypes.c_void_p(_ptr(self._dev_vals)),
+                                               float(diag_val), ctypes.c_void_p(stream())),
+                  'zero_rows')
+        torch.cuda.current_stream().synchronize()
+
+    def _assemble(self):
+        pass
+
+    @property
+    def array(self):
+        """Host copy of the CSR values."""
+        self._check_err()
+        return self._dev_vals[:self._sparsity.nz].cpu().numpy()
+
+    @property
+    def values(self):
+        """Dense host copy of the matrix (small matrices only)."""
+        sp = self._sparsity
+        n = sp.nrows * self.dims[0]
+        m = sp.ncols * self.dims[1]
+        out = np.zeros((n, m), dtype=np.float64)
+        vals = self.array
+        rp, ci = sp._rowptr, sp._colidx
+        for r in range(n):
+            out[r, ci[rp[r]:rp[r + 1]]] = vals[rp[r]:rp[r + 1]]
+        return out
+
+    @property
+    def handle(self):
+        return self
+
+
+class Solver(base.Solver):
+    """Jacobi-preconditioned CG on the device with PETSc KSPCG semantics
+    (reference: petsc_base.py:160-220)."""
+
+    REASONS = {2: 'CONVERGED_RTOL', 3: 'CONVERGED_ATOL', -3: 'DIVERGED_ITS',
+               -4: 'DIVERGED_DTOL', -8: 'DIVERGED_INDEFINITE_MAT', -9: 'DIVERGED_NANORINF'}
+
+    def solve(self, A, x, b):
+        p = self.parameters
+        if p['linear_solver'] != 'cg':
+            raise NotImplementedError("linear_solver %r: the hip backend implements 'cg'"
+                                      % p['linear_solver'])
+        if p['preconditioner'] not in ('jacobi',):
+            raise NotImplementedError("preconditioner %r: the hip backend implements 'jacobi'"
+                                      % p['preconditioner'])
+        if p['plot_convergence']:
+            p['monitor_convergence'] = True
+        A._check_err()
+        sp = A.sparsity
+        n = sp.nrows * A.dims[0]
+        bdev = b._to_device()
+        x._to_device()
+        work = torch.empty(int(nat.lib.op2hip_pcg_work_size(n)), dtype=torch.float64,
+                           device=device())
+        maxit = int(p['maximum_iterations'])
+        hist = np.zeros(maxit + 1) if p['monitor_convergence'] else None
+        its = ctypes.c_int()
+        rnorm = ctypes.c_double()
+        reason = ctypes.c_int()
+        nat.check(nat.lib.op2hip_pcg(
+            n, sp.nz, ctypes.c_void_p(_ptr(sp._dev_rowptr)), ctypes.c_void_p(_ptr(sp._dev_colidx)),
+            ctypes.c_void_p(_ptr(A._dev_vals)), ctypes.c_void_p(_ptr(bdev)),
+            ctypes.c_void_p(_ptr(x._dev)), float(p['relative_tolerance']),
+            float(p['absolute_tolerance']), float(p['divergence_tolerance']), maxit,
+            ctypes.c_void_p(_ptr(work)),
+            hist.ctypes.data_as(ctypes.c_void_p) if hist is not None else None,
+            int(p.get('hip_cg_chunk', 16)), ctypes.byref(its), ctypes.byref(rnorm),
+            ctypes.byref(reason), ctypes.c_void_p(stream())), 'pcg')
+        x._device_written()
+        x.needs_halo_update = True
+        self._its, self._rnorm, self._reason = its.value, rnorm.value, reason.value
+        if hist is not None:
+            self.reshist = list(hist[:its.value + 1])
+        debug("Converged reason: %s" % self.REASONS.get(reason.value, reason.value))
+        debug("Iterations: %s" % its.value)
+        debug("Residual norm: %s" % rnorm.value)
+        if reason.value < 0:
+            msg = "KSP Solver failed to converge in %d iterations: %s (Residual norm: %e)" \
+                % (its.value, self.REASONS.get(reason.value, reason.value), rnorm.value)
+            if p['error_on_nonconvergence']:
+                raise RuntimeError(msg)
+            warning(msg)
+
+    def getIterationNumber(self):
+        return self._its
+
+    def getResidualNorm(self):
+        return self._rnorm
+
+    def getConvergedReason(self):
+        return self._reason
+
+
+# -- plans -------------------------------------------------------------------
+
+class Plan(object):
+    """Partition and colouring of an iteration set for a coloured loop,
+    built natively (csrc/plan.cpp) and kept on the device.
+
+    Properties mirror the reference's plan (pyop2/plan.pyx:378-455):
+    ``nblocks``, ``ncolors``, ``ncolblk``, ``blkmap``, ``offset``,
+    ``nelems``, ``nthrcol``, ``thrcol``; ``elems`` is the execution order."""
+
+    ARRAYS = ('elems', 'offset', 'nelems', 'thrcol', 'nthrcol', 'blkcol', 'blkmap', 'ncolblk')
+
+    def __init__(self, n, part_size, reorder, race):
+        self.n = n
+        self.part_size = part_size
+        self.race = race  # [(map values, dim, idx, space, space_size)]
+        spaces = []
+        for r in race:
+            if r[3] not in spaces:
+                spaces.append(r[3])
+        sizes = [0] * len(spaces)
+        for r in race:
+            sizes[spaces.index(r[3])] = r[4]
+        self._maps_keep = [np.ascontiguousarray(r[0], dtype=np.int32) for r in race]
+        self._args = (nat.ptr_array([m.ctypes.data for m in self._maps_keep]),
+                      nat.int_array([r[1] for r in race]),
+                      nat.int_array([r[2] for r in race]),
+                      nat.int_array([spaces.index(r[3]) for r in race]), len(spaces),
+                      nat.int_array(sizes))
+        h = ctypes.c_void_p()
+        mp, dims, idxs, sp, nsp, ssz = self._args
+        nat.check(nat.lib.op2hip_plan_build(n, part_size, 1 if reorder else 0, len(race), mp, dims,
+                                            idxs, sp, nsp, ssz, ctypes.byref(h)), 'plan build')
+        try:
+            v = ctypes.c_int64()
+            nat.check(nat.lib.op2hip_plan_scalar(h, 1, ctypes.byref(v)))
+            self.nblocks = v.value
+            nat.check(nat.lib.op2hip_plan_scalar(h, 2, ctypes.byref(v)))
+            self.ncolors = v.value
+            sizes = {'elems': n, 'offset': self.nblocks, 'nelems': self.nblocks, 'thrcol': n,
+                     'nthrcol': self.nblocks, 'blkcol': self.nblocks, 'blkmap': self.nblocks,
+                     'ncolblk': self.ncolors}
+            for k, name in enumerate(self.ARRAYS):
+                a = np.zeros(sizes[name], dtype=np.int32)
+                if a.size:
+                    nat.check(nat.lib.op2hip_plan_array(h, k, a.ctypes.data_as(ctypes.c_void_p)))
+                setattr(self, name, a)
+        finally:
+            nat.lib.op2hip_plan_destroy(h)
+        self.nthrcol_max = int(self.nthrcol.max()) if self.nblocks else 0
+        self.colour_start = np.concatenate([[0], np.cumsum(self.ncolblk)]).astype(np.int64)
+        self._dev = {}
+        for name in ('elems', 'blkmap', 'offset', 'nelems', 'nthrcol', 'thrcol'):
+            a = getattr(self, name)
+            d = _dev_bytes(a.nbytes)
+            _h2d(d, a)
+            self._dev[name] = d
+
+    def verify(self):
+        """True when the plan is conflict-free for its race maps."""
+        ok = ctypes.c_int()
+        mp, dims, idxs, sp, nsp, ssz = self._args
+        c = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        nat.check(nat.lib.op2hip_plan_verify(self.n, self.nblocks, c(self.elems), c(self.offset),
+                                             c(self.nelems), c(self.thrcol), c(self.blkcol),
+                                             len(self.race), mp, dims, idxs, sp, nsp, ssz,
+                                             ctypes.byref(ok)), 'plan verify')
+        return bool(ok.value)
+
+
+_plan_cache = {}
+
+
+def _race_maps(args):
+    """Race targets of a loop: (map, idx, space key) per conflicting write.
+
+    INC through a map and every Mat (its row map) need conflict handling,
+    as in the reference plan (pyop2/plan.pyx:225-238 with matrix colouring)."""
+    race = []
+    for a in args:
+        if a._is_mat:
+            rmap = a.map[0]
+            race.append((rmap, -1, ('mat', id(rmap.dataset)), rmap.dataset.total_size))
+        elif a._is_indirect and a.access is INC:
+            idx = a.idx if isinstance(a.idx, int) else -1
+            race.append((a.map, idx, ('dat', id(a.map.dataset)), a.map.dataset.total_size))
+    return race
+
+
+def get_plan(iterset, n, args, part_size=None, reorder=None):
+    """The cached plan of a coloured loop over ``n`` elements of ``iterset``."""
+    if part_size is None:
+        part_size = cfg.get('hip_partition_size') or 256
+    if reorder is None:
+        reorder = cfg.get('hip_reorder', True)
+    race = _race_maps(args)
+    # one entry per distinct (map, idx, space)
+    seen, uniq = set(), []
+    for r in race:
+        k = (id(r[0]), r[1], r[2])
+        if k not in seen:
+            seen.add(k)
+            uniq.append(r)
+    key = (id(iterset), n, part_size, bool(reorder), tuple((id(r[0]), r[1], r[2]) for r in uniq))
+    p = _plan_cache.get(key)
+    if p is None:
+        p = Plan(n, part_size, reorder,
+                 [(r[0].values[:n], r[0].dim, r[1], r[2], r[3]) for r in uniq])
+        p._keep = [r[0] for r in uniq]
+        _plan_cache[key] = p
+    return p
+
+
+# -- loops -------------------------------------------------------------------
+
+class _CompiledLoop(object):
+
+    def __init__(self, source):
+        self.source = source
+        self.module = jit.Module(jit.compile_source(source.source()))
+        self.fn = self.module.function(source.entry)
+        self.reduce_fn = self.module.function(source.reduce_entry) if source.reduce_entry else None
+        self.const_versions = {}
+
+
+_loop_cache = {}
+_scratch = {}
+
+
+def _partials(pos, nbytes):
+    t = _scratch.get(pos)
+    if t is None or t.numel() < nbytes:
+        t = _dev_bytes(max(nbytes, 4096))
+        _scratch[pos] = t
+    return t
+
+
+class JITModule(base.JITModule):
+    """Cache key holder of compiled loops (key as the reference's JITModule)."""
+    _cache = {}
+
+    def __init__(self, kernel, itspace_extents, *args, **kwargs):
+        pass
+
+
+def _block_size():
+    return cfg.get('hip_block_size') or 256
+
+
+class ParLoop(base.ParLoop):
+
+    def _mode(self):
+        return 'coloured' if any(a._is_mat or (a._is_indirect and a.access is INC)
+                                 for a in self.args) else 'stream'
+
+    def _compiled(self, mode, bs, consts):
+        key = base.JITModule._cache_key(self.kernel, self._it_space.extents, *self.args)
+        key = key + (mode, bs)
+        c = _loop_cache.get(key)
+        if c is None:
+            src = LoopSource(self.kernel, self._it_space.extents, self.args, consts, mode, bs)
+            c = _CompiledLoop(src)
+            _loop_cache[key] = c
+        return c
+
+    def compute(self):
+        n = self._it_space.exec_size if self.needs_exec_halo else self._it_space.size
+        mode = self._mode()
+        bs = _block_size()
+        consts = base.Const._definitions()
+        loop = self._compiled(mode, bs, consts)
+        s = stream()
+        # constants
+        for c in consts:
+            if loop.const_versions.get(c.name) != (id(c), c._version):
+                ptr, nbytes = loop.module.global_symbol(c.name)
+                data = np.ascontiguousarray(c.data)
+                nat.check(nat.lib.op2hip_set_global(ptr, data.ctypes.data_as(ctypes.c_void_p),
+                                                    min(nbytes, data.nbytes), ctypes.c_void_p(s)),
+                          'const upload')
+                loop.const_versions[c.name] = (id(c), c._version)
+        # arguments to the device
+        for a in self.args:
+            if a._is_dat or a._is_global:
+                a.data._to_device()
+        plan = None
+        if mode == 'coloured':
+            plan = get_plan(self._it_space.iterset, n, self.args,
+                            part_size=cfg.get('hip_partition_size') or bs)
+            nparts = plan.nblocks
+            grid_stream = 0
+        else:
+            grid_stream = max(1, min((n + bs - 1) // bs, 4096))
+            nparts = grid_stream
+        partials = {}
+        argv_args = []
+        for role, pos in loop.source.param_roles():
+            a = self.args[pos] if pos is not None else None
+            if role in ('start', 'count') or role.startswith('plan_'):
+                argv_args.append(role)
+            elif role == 'gbl':
+                argv_args.append(_ptr(a.data._dev))
+            elif role == 'partial':
+                t = _partials(pos, nparts * a.data.cdim * a.data.dtype.itemsize)
+                partials[pos] = t
+                argv_args.append(_ptr(t))
+            elif role == 'data':
+                argv_args.append(_ptr(a.data._dev))
+            elif role == 'map':
+                argv_args.append(_ptr(a.map._device_values()))
+            elif role == 'mat_vals':
+                argv_args.append(_ptr(a.data._dev_vals))
+            elif role == 'mat_rowptr':
+                argv_args.append(_ptr(a.data.sparsity._dev_rowptr))
+            elif role == 'mat_colidx':
+                argv_args.append(_ptr(a.data.sparsity._dev_colidx))
+            elif role == 'mat_rmap':
+                argv_args.append(_ptr(a.map[0]._device_values()))
+            elif role == 'mat_cmap':
+                argv_args.append(_ptr(a.map[1]._device_values()))
+            elif role == 'err_flag':
+                mats = [x for x in self.args if x._is_mat]
+                argv_args.append(_ptr(mats[0].data._err))
+            else:
+                raise RuntimeError("unknown parameter role %s" % role)
+        if n > 0:
+            if mode == 'stream':
+                argv = [0 if r == 'start' else n if r == 'count' else r for r in argv_args]
+                jit.launch(loop.fn, grid_stream, bs, s, argv)
+            else:
+                pdev = plan._dev
+                for col in range(plan.ncolors):
+                    cnt = int(plan.ncolblk[col])
+                    if cnt == 0:
+                        continue
+                    argv = []
+                    for r in argv_args:
+                        if r == 'start':
+                            argv.append(int(plan.colour_start[col]))
+                        elif r == 'count':
+                            argv.append(cnt)
+                        elif r == 'plan_elems':
+                            argv.append(_ptr(pdev['elems']))
+                        elif r == 'plan_blkmap':
+                            argv.append(_ptr(pdev['blkmap']))
+                        elif r == 'plan_offset':
+                            argv.append(_ptr(pdev['offset']))
+                        elif r == 'plan_nelem':
+                            argv.append(_ptr(pdev['nelems']))
+                        elif r == 'plan_nthrcol':
+                            argv.append(_ptr(pdev['nthrcol']))
+                        elif r == 'plan_thrcol':
+                            argv.append(_ptr(pdev['thrcol']))
+                        else:
+                            argv.append(r)
+                    jit.launch(loop.fn, cnt, bs, s, argv)
+            if loop.reduce_fn is not None:
+                rargv = []
+                for role, pos in loop.source.reduce_roles():
+                    if role == 'nparts':
+                        rargv.append(nparts)
+                    elif role == 'gbl_out':
+                        rargv.append(_ptr(self.args[pos].data._dev))
+                    else:
+                        rargv.append(_ptr(partials[pos]))
+                jit.launch(loop.reduce_fn, 1, 256, s, rargv)
+        # coherence: written arguments are newer on the device; stale host
+        # views become read-only (reference: sequential.py:85-86)
+        for a in self.args:
+            if a._is_dat:
+                if a.access is not READ:
+                    a.data._device_written()
+                maybe_setflags(a.data._data, write=False)
+            elif a._is_global_reduction:
+                a.data._device_written()
+        self.maybe_set_halo_update_needed()
+        for a in self.args:
+            if a._is_mat:
+                a.data._assemble()
+
+
+def par_loop(kernel, it_space, *args):
+    """Run ``kernel`` over ``it_space`` (reference: op2.py:125-158)."""
+    ParLoop(kernel, it_space, *args).compute()
+
+
+@validate_type(('M', base.Mat, MatTypeError), ('x', base.Dat, DatTypeError),
+               ('b', base.Dat, DatTypeError))
+def solve(M, x, b):
+    Solver().solve(M, x, b)
