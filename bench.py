@@ -1,0 +1,175 @@
+"""Benchmark: airfoil finite-volume time steps on one MI355X.
+
+Workload: the 1200 x 600 synthetic quad mesh (720,000 cells, 1,438,200
+interior edges, 3,600 boundary edges, numbering permuted).  A step is one
+airfoil iteration: save_soln, then 2 x (adt_calc, res_calc, bres_calc,
+update with the rms Global).  ``value`` is algorithmic (unique) bytes per
+step x steps / wall time of the timed region, inputs resident in HBM.
+
+``roofline`` is for res_calc, the dominant loop: its unique bytes per call
+divided by its average duration, timed with HIP events on the stream the
+loop is launched on.  ``cpu_baseline`` times the CPU oracle (the
+reference's sequential backend restated, one core) on a bounded number of
+steps of the same mesh.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures
+one GPU.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=50)
+    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--nx', type=int, default=1200)
+    p.add_argument('--ny', type=int, default=600)
+    p.add_argument('--graph', type=int, default=1, help='replay each step as a hipGraph')
+    p.add_argument('--cpu-seconds', type=float, default=15.0,
+                   help='approximate CPU time budget of the oracle baseline (0: skip)')
+    p.add_argument('--partition-size', type=int, default=0)
+    p.add_argument('--block-size', type=int, default=0)
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; --gpus %d is not supported (every run uses one "
+              "MI355X)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from pyop2_amd import op2, hip
+    from pyop2_amd import _native as nat
+    from pyop2_amd.workloads import airfoil, meshes
+
+    opts = {'backend': 'hip'}
+    if args.partition_size:
+        opts['hip_partition_size'] = args.partition_size
+    if args.block_size:
+        opts['hip_block_size'] = args.block_size
+    op2.init(**opts)
+    mesh = meshes.QuadMesh(args.nx, args.ny, seed=0)
+    A = airfoil.Airfoil(op2, mesh)
+    ub = airfoil.unique_bytes(mesh)
+    step_bytes = ub['save_soln'] + 2 * (ub['adt_calc'] + ub['res_calc'] + ub['bres_calc'] +
+                                        ub['update'])
+    s = hip.stream()
+    for _ in range(args.warmup):
+        A.iteration()
+    torch.cuda.synchronize()
+
+    step = A.iteration
+    graph = None
+    if args.graph:
+        graph = hip.capture(A.iteration)
+        graph.replay()
+        torch.cuda.synchronize()
+        step = graph.replay
+
+    # events around every res_calc call of the timed region
+    ev = []
+
+    def mkev():
+        e = ctypes.c_void_p()
+        nat.check(nat.lib.op2hip_event_create(ctypes.byref(e)))
+        return e
+
+    t_start, t_stop = mkev(), mkev()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nat.lib.op2hip_event_record(t_start, ctypes.c_void_p(s))
+    for _ in range(args.steps):
+        step()
+    nat.lib.op2hip_event_record(t_stop, ctypes.c_void_p(s))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ctypes.c_float()
+    nat.lib.op2hip_event_elapsed_ms(t_start, t_stop, ctypes.byref(ms))
+    gpu_s = ms.value / 1e3
+    elapsed = max(wall, gpu_s)
+
+    # res_calc alone, same stream, events around each call (outside the graph)
+    nres = max(10, args.steps)
+    pairs = [(mkev(), mkev()) for _ in range(nres)]
+    for a, b in pairs:
+        nat.lib.op2hip_event_record(a, ctypes.c_void_p(s))
+        A.res_calc()
+        nat.lib.op2hip_event_record(b, ctypes.c_void_p(s))
+    torch.cuda.synchronize()
+    res_ms = []
+    for a, b in pairs:
+        nat.lib.op2hip_event_elapsed_ms(a, b, ctypes.byref(ms))
+        res_ms.append(ms.value)
+    res_avg_s = float(np.mean(res_ms)) / 1e3
+    res_gbs = ub['res_calc'] / res_avg_s / 1e9
+
+    value = step_bytes * args.steps / elapsed / 1e9
+    out = {
+        'metric': 'par_loop HBM GB/s (% of peak)',
+        'value': round(value, 2),
+        'unit': 'GB/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(elapsed / args.steps * 1e3, 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': 'airfoil %dx%d quad mesh, permuted numbering, '
+                               'save_soln + 2x(adt_calc, res_calc, bres_calc, update) per step'
+                               % (args.nx, args.ny),
+                   'cells': mesh.ncell, 'edges': mesh.nedge, 'bedges': mesh.nbedge,
+                   'unique_bytes_per_step': int(step_bytes),
+                   'pct_of_hbm_peak': round(100.0 * value / HBM_PEAK_GBS, 2),
+                   'graph': bool(args.graph),
+                   'partition_size': hip.cfg.get('hip_partition_size') or 'auto'},
+        'roofline': {'bound': 'hbm', 'kernel': 'op2_res_calc_coloured (one res_calc par_loop)',
+                     'achieved': round(res_gbs, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(res_gbs / HBM_PEAK_GBS, 4),
+                     'avg_us': round(res_avg_s * 1e6, 2),
+                     'bytes_per_launch': int(ub['res_calc']), 'traffic': None},
+    }
+    if args.cpu_seconds > 0:
+        out['cpu_baseline'] = cpu_baseline(mesh, args.cpu_seconds, step_bytes)
+    A.remove_consts()
+    print(json.dumps(out))
+
+
+def cpu_baseline(mesh, budget_s, step_bytes):
+    """The CPU oracle (one core) on whole steps of the same mesh."""
+    from oracle import sequential as seq
+    from pyop2_amd.workloads import airfoil
+    B = airfoil.Airfoil(seq, mesh, consts=False)
+    B.iteration()  # compile the wrappers
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        B.iteration()
+        n += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {'value': round(step_bytes * n / dt / 1e9, 3), 'unit': 'GB/s', 'cores': 1,
+            'kind': 'port',
+            'sample': '%d airfoil steps on the same mesh, sequential oracle (gcc -O2), '
+                      '%.1f ms/step' % (n, dt / n * 1e3)}
+
+
+if __name__ == '__main__':
+    main()

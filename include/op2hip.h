@@ -1,0 +1,144 @@
+/* op2hip: C ABI of the MI355X (gfx950) backend of the op2 programming model.
+ *
+ * Library: pyop2_amd/lib/libop2hip.so (built by `python -m pyop2_amd.build`).
+ * Python binds it with ctypes (pyop2_amd/_native.py); any FFI can.
+ *
+ * Conventions
+ *   - every call returns int status: 0 ok, < 0 error; op2hip_last_error()
+ *     returns the thread-local message of the last failure.  No C++
+ *     exception crosses the ABI.
+ *   - device buffers are passed as plain pointers and are owned by the
+ *     caller; opaque handles (modules, plans, sparsity builds) have an
+ *     explicit destroy call.
+ *   - every call that enqueues device work takes a hipStream_t as void*.
+ *   - one host thread per handle; handles are not thread-safe.
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the reference repository gbts/PyOP2).
+ */
+#ifndef OP2HIP_H
+#define OP2HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime --------------------------------------------------------------- */
+
+/* Message of the last failing call on this thread. */
+const char* op2hip_last_error(void);
+
+/* Select and initialise GPU `device`.
+ * Replaces: op_init of OP2-Common, pyop2/op_lib_core.pyx:131-154 (called
+ * from op2.init, pyop2/op2.py:82). */
+int op2hip_init(int device);
+
+/* gcnArchName of `device` into `out`. */
+int op2hip_device_arch(int device, char* out, int outlen);
+
+/* ---- JIT of generated loops ---------------------------------------------------
+ * Replaces the compile step of a JITModule: instant.inline_with_numpy in
+ * pyop2/host.py:225-261 and pycuda SourceModule in pyop2/cuda.py:602-651.
+ * op2hip_compile needs no GPU; it keeps an on-disk code-object cache in
+ * `cache_dir` (may be NULL) keyed by source, options and hiprtc version. */
+int op2hip_compile(const char* src, const char* const* opts, int nopts, const char* cache_dir,
+                   void** code_out, size_t* code_size, int* cache_hit);
+void op2hip_free_host(void* p);
+int op2hip_module_load(const void* code, size_t size, void** mod_out);
+int op2hip_module_destroy(void* mod);
+int op2hip_module_function(void* mod, const char* name, void** fn_out);
+/* Device address and size of a module global (a kernel Const).
+ * Replaces the per-call Const copy-in of pyop2/host.py:271-277. */
+int op2hip_module_global(void* mod, const char* name, void** dptr, size_t* bytes);
+/* Copy `bytes` host bytes to a module global on `stream`, then wait. */
+int op2hip_set_global(void* dptr, const void* src, size_t bytes, void* stream);
+
+/* Launch a generated loop kernel whose parameters are all 8 bytes wide:
+ * argv[i] is parameter i (pointer or 64-bit integer).
+ * Replaces the wrapper call fun(start, end, args...) of
+ * pyop2/sequential.py:96-117 and the per-colour launches of
+ * pyop2/cuda.py:757-800. */
+int op2hip_launch(void* fn, unsigned grid, unsigned block, unsigned lds, void* stream,
+                  const uint64_t* argv, int nargs);
+int op2hip_function_attrs(void* fn, int* num_regs, int* local_bytes, int* max_threads);
+
+/* Stream-ordered timing. */
+int op2hip_event_create(void** ev);
+int op2hip_event_destroy(void* ev);
+int op2hip_event_record(void* ev, void* stream);
+int op2hip_event_elapsed_ms(void* start, void* stop, float* ms);
+int op2hip_stream_sync(void* stream);
+
+/* ---- plan ----------------------------------------------------------------------
+ * Replaces pyop2/plan.pyx:68-456 (Plan: partition, staging, two-level
+ * colouring; properties plan.pyx:378-455).  Race maps are host int32 arrays
+ * (row-major, dims[i] columns, at least nelems rows); idxs[i] the column
+ * used (-1 all); spaces[i] the target index space of map i, of size
+ * space_size[spaces[i]].  reorder = 1 orders the iteration set into compact
+ * regions grown breadth-first through the race maps before partitioning. */
+int op2hip_plan_build(int nelems, int part_size, int reorder, int nmaps, const int* const* maps,
+                      const int* dims, const int* idxs, const int* spaces, int nspaces,
+                      const int* space_size, void** plan_out);
+/* field: 0 nelems, 1 nblocks, 2 ncolors, 3 max block size, 4 max thread colours */
+int op2hip_plan_scalar(void* plan, int field, int64_t* out);
+/* which: 0 elems, 1 offset, 2 nelems, 3 thrcol, 4 nthrcol, 5 blkcol, 6 blkmap,
+ * 7 ncolblk (host int32 destination of the array's length) */
+int op2hip_plan_array(void* plan, int which, int* dst);
+int op2hip_plan_destroy(void* plan);
+/* Conflict-freedom check (the invariant of test/unit/test_coloring.py:104-112). */
+int op2hip_plan_verify(int nelems, int nblocks, const int* elems, const int* offset,
+                       const int* nelem, const int* thrcol, const int* blkcol, int nmaps,
+                       const int* const* maps, const int* dims, const int* idxs,
+                       const int* spaces, int nspaces, const int* space_size, int* ok);
+
+/* ---- sparsity ------------------------------------------------------------------
+ * Replaces build_sparsity_pattern_seq (pyop2/sparsity_utils.h:10-13,
+ * sparsity_utils.cxx:5-51) as called by build_sparsity
+ * (pyop2/op_lib_core.pyx:544-596).  Two phases so the caller allocates:
+ * build writes rowptr (device, nrows*rmult+1) and *nnz_out; fill writes
+ * colidx (device, nnz) and nnz_rows (device, nrows*rmult, may be NULL) and
+ * releases the handle.  rowmaps/colmaps: host arrays of device int32 map
+ * pointers; nelems[m]: iteration-set size of pair m.  Bit-exact to the
+ * reference's sorted-set construction. */
+int op2hip_sparsity_build(int rmult, int cmult, int nrows, int nmaps, const int* const* rowmaps,
+                          const int* rdims, const int* const* colmaps, const int* cdims,
+                          const long long* nelems, int* rowptr, void** handle,
+                          long long* nnz_out, void* stream);
+int op2hip_sparsity_fill(void* handle, const int* rowptr, int* colidx, int* nnz_rows,
+                         void* stream);
+int op2hip_sparsity_discard(void* handle);
+
+/* ---- matrices and solver ---------------------------------------------------------
+ * Mat storage is FP64 CSR on the Sparsity (pyop2/petsc_base.py:80-120,
+ * createAIJWithArrays); element insertion (addto_scalar/addto_vector,
+ * pyop2/mat_utils.h:8-10) happens inside the generated loops. */
+
+/* y = A x.  Replaces PETSc MatMult inside KSP. */
+int op2hip_spmv(int n, long long nnz, const int* rowptr, const int* colidx, const double* vals,
+                const double* x, double* y, void* stream);
+
+/* Zero rows except the diagonal, set to diag.
+ * Replaces Mat.zero_rows -> MatZeroRowsLocal (pyop2/petsc_base.py:131-135). */
+int op2hip_csr_zero_rows(int nlist, const int* rows, const int* rowptr, const int* colidx,
+                         double* vals, double diag, void* stream);
+
+/* Jacobi-preconditioned CG from x = 0, PETSc KSPCG semantics.
+ * Replaces Solver.solve (pyop2/petsc_base.py:182-220; parameters
+ * pyop2/base.py:1591-1601).  work: device doubles of
+ * op2hip_pcg_work_size(n); reshist_host: optional max_it+1 doubles;
+ * reason: PETSc KSPConvergedReason (2 rtol, 3 atol, -3 its, -4 dtol,
+ * -8 indefinite, -9 nan). */
+int op2hip_pcg(int n, long long nnz, const int* rowptr, const int* colidx, const double* vals,
+               const double* b, double* x, double rtol, double atol, double dtol, int maxit,
+               double* work, double* reshist_host, int chunk, int* its_out, double* rnorm_out,
+               int* reason_out, void* stream);
+long long op2hip_pcg_work_size(int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OP2HIP_H */

@@ -90,10 +90,11 @@ class Set(base.Set):
 
 
 class Map(base.Map):
-    pass
+    _arg_class = Arg
 
 
 class Dat(base.Dat):
+    _arg_class = Arg
 
     def zero(self):
         k = Kernel("void zero(%(t)s *dat) { for (int n = 0; n < %(d)d; ++n) dat[n] = (%(t)s)0; }"
@@ -102,7 +103,7 @@ class Dat(base.Dat):
 
 
 class Global(base.Global):
-    pass
+    _arg_class = Arg
 
 
 class Const(base.Const):
@@ -146,6 +147,7 @@ class Sparsity(base.Sparsity):
 
 
 class Mat(base.Mat):
+    _arg_class = Arg
 
     def __init__(self, *args, **kwargs):
         base.Mat.__init__(self, *args, **kwargs)

@@ -24,6 +24,16 @@ from .utils import (as_tuple, as_type, maybe_setflags, preprocess, validate_in,
 from . import configuration as cfg
 
 
+def _new_arg(owner, **kwargs):
+    """Build an :class:`Arg` for ``owner``: its class's own ``_arg_class``
+    when it names one (backends usable without op2.init), else the selected
+    backend's Arg."""
+    cls = getattr(owner, '_arg_class', None)
+    if cls is not None:
+        return cls(**kwargs)
+    return _make_object('Arg', **kwargs)
+
+
 # Access descriptors (reference: base.py:56-90)
 
 class Access(object):
@@ -409,7 +419,7 @@ class Dat(DataCarrier):
         if isinstance(path, Map):
             if path._dataset != self._dataset and path != IdentityMap:
                 raise MapValueError("Dataset of Map does not match Dataset of Dat.")
-            return _make_object('Arg', data=self, map=path, access=access)
+            return _new_arg(self, data=self, map=path, access=access)
         else:
             path._dat = self
             path._access = access
@@ -618,7 +628,7 @@ class Global(DataCarrier):
 
     @validate_in(('access', _modes, ModeValueError))
     def __call__(self, access):
-        return _make_object('Arg', data=self, access=access)
+        return _new_arg(self, data=self, access=access)
 
     def __str__(self):
         return "OP2 Global Argument: %s with dim %s and value %s" \
@@ -705,7 +715,7 @@ class Map(object):
             raise IndexValueError("Index must be in interval [0,%d]" % (self._dim - 1))
         if isinstance(index, IterationIndex) and index.index not in [0, 1]:
             raise IndexValueError("IterationIndex must be in interval [0,1]")
-        return _make_object('Arg', map=self, idx=index)
+        return _new_arg(self, map=self, idx=index)
 
     def __iter__(self):
         yield self
@@ -892,7 +902,7 @@ class Mat(DataCarrier):
         path_idxs = [arg.idx for arg in path]
         if tuple(path_maps) not in [tuple(p) for p in self.sparsity.maps]:
             raise MapValueError("Path maps not in sparsity maps")
-        return _make_object('Arg', data=self, map=path_maps, access=access, idx=path_idxs)
+        return _new_arg(self, data=self, map=path_maps, access=access, idx=path_idxs)
 
     @property
     def dims(self):

@@ -148,8 +148,9 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         self.consts = consts
         self.mode = mode
         self.bs = block_size
-        self.entry = 'op2_loop'
-        self.reduce_entry = 'op2_reduce' if self.reductions else None
+        # kernel names carry the user kernel's name so profiles tell loops apart
+        self.entry = 'op2_%s_%s' % (kernel.name, mode)
+        self.reduce_entry = ('op2_%s_reduce' % kernel.name) if self.reductions else None
 
     @property
     def reductions(self):
@@ -425,8 +426,8 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         for a in self.reductions:
             decl += ['%s* __restrict__ g%d' % (a.ctype, a.pos),
                      'const %s* __restrict__ r%d' % (a.ctype, a.pos)]
-        lines = ['extern "C" __global__ void __launch_bounds__(256) op2_reduce(%s) {'
-                 % ', '.join(decl)]
+        lines = ['extern "C" __global__ void __launch_bounds__(256) %s(%s) {'
+                 % (self.reduce_entry, ', '.join(decl))]
         for a in self.reductions:
             op = _mode(a.access)
             expr = _REDUCE_OPS[op][0]

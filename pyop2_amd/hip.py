@@ -395,12 +395,18 @@ class Plan(object):
             nat.lib.op2hip_plan_destroy(h)
         self.nthrcol_max = int(self.nthrcol.max()) if self.nblocks else 0
         self.colour_start = np.concatenate([[0], np.cumsum(self.ncolblk)]).astype(np.int64)
-        self._dev = {}
-        for name in ('elems', 'blkmap', 'offset', 'nelems', 'nthrcol', 'thrcol'):
-            a = getattr(self, name)
-            d = _dev_bytes(a.nbytes)
-            _h2d(d, a)
-            self._dev[name] = d
+        self._dev = None
+
+    def device_arrays(self):
+        """The plan arrays the coloured loops read, uploaded on first use."""
+        if self._dev is None:
+            self._dev = {}
+            for name in ('elems', 'blkmap', 'offset', 'nelems', 'nthrcol', 'thrcol'):
+                a = getattr(self, name)
+                d = _dev_bytes(a.nbytes)
+                _h2d(d, a)
+                self._dev[name] = d
+        return self._dev
 
     def verify(self):
         """True when the plan is conflict-free for its race maps."""
@@ -574,7 +580,7 @@ class ParLoop(base.ParLoop):
                 argv = [0 if r == 'start' else n if r == 'count' else r for r in argv_args]
                 jit.launch(loop.fn, grid_stream, bs, s, argv)
             else:
-                pdev = plan._dev
+                pdev = plan.device_arrays()
                 for col in range(plan.ncolors):
                     cnt = int(plan.ncolblk[col])
                     if cnt == 0:
@@ -634,3 +640,25 @@ def par_loop(kernel, it_space, *args):
                ('b', base.Dat, DatTypeError))
 def solve(M, x, b):
     Solver().solve(M, x, b)
+
+
+class Graph(object):
+    """A captured sequence of op2 calls, replayed as one hipGraph.
+
+    ``fn`` must have run eagerly before (so loops are compiled, plans built
+    and scratch allocated) and must not touch host data (``.data``) or change
+    Consts.  Replaying re-executes every captured launch on the current
+    stream; host-side coherence states are those left by the capture."""
+
+    def __init__(self, fn):
+        self._g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self._g):
+            fn()
+
+    def replay(self):
+        self._g.replay()
+
+
+def capture(fn):
+    """Capture ``fn`` (a sequence of par_loops) into a :class:`Graph`."""
+    return Graph(fn)

@@ -1,0 +1,97 @@
+"""Plan invariants (host-built, no GPU needed).
+
+The reference pins its plan by the conflict-freedom invariant
+(test/unit/test_coloring.py:104-112) and structural properties
+(plan.pyx:107-110, 374-376): every element in exactly one block, ncolblk
+sums to nblocks, blkmap is a permutation sorted by colour, no two elements
+of one (block, thread colour) and no two blocks of one colour share an INC
+target.  These are checked here with numpy, independently of the native
+verifier, on random maps (hypothesis) and on permuted airfoil meshes."""
+
+import numpy as np
+import pytest
+from hypothesis import given, settings, strategies as st
+
+from pyop2_amd.hip import Plan
+from pyop2_amd.workloads import meshes
+
+
+def check_plan(p, maps, n):
+    """maps: list of (values[n, dim], idx, space) race maps."""
+    assert sorted(p.elems.tolist()) == list(range(n))
+    assert p.nelems.sum() == n
+    assert (p.offset == np.concatenate([[0], np.cumsum(p.nelems)[:-1]])).all()
+    assert p.ncolblk.sum() == p.nblocks
+    assert sorted(p.blkmap.tolist()) == list(range(p.nblocks))
+    assert (np.diff(p.blkcol[p.blkmap]) >= 0).all()
+    # per-element target lists
+    def targets(e):
+        out = set()
+        for v, idx, space in maps:
+            cols = [idx] if idx >= 0 else range(v.shape[1])
+            for c in cols:
+                out.add((space, int(v[e, c])))
+        return out
+    owner_blockcol = {}
+    for b in range(p.nblocks):
+        els = p.elems[p.offset[b]:p.offset[b] + p.nelems[b]]
+        cols = p.thrcol[p.offset[b]:p.offset[b] + p.nelems[b]]
+        assert cols.max() < p.nthrcol[b] if len(cols) else True
+        seen = {}
+        blk_targets = set()
+        for e, c in zip(els, cols):
+            for t in targets(e):
+                assert seen.get((t, c)) in (None, e), "thread colour conflict"
+                seen[(t, c)] = e
+                blk_targets.add(t)
+        for t in blk_targets:
+            key = (t, p.blkcol[b])
+            assert owner_blockcol.get(key) in (None, b), "block colour conflict"
+            owner_blockcol[key] = b
+    assert p.verify()
+
+
+@settings(max_examples=25, deadline=None)
+@given(n=st.integers(1, 600), ntarget=st.integers(1, 80), dim=st.integers(1, 4),
+       ps=st.sampled_from([8, 32, 64, 256]), reorder=st.booleans(), seed=st.integers(0, 1000))
+def test_random_maps(n, ntarget, dim, ps, reorder, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.integers(0, ntarget, size=(n, dim)).astype(np.int32)
+    race = [(v, dim, -1, 'a', ntarget)]
+    p = Plan(n, ps, reorder, race)
+    check_plan(p, [(v, -1, 'a')], n)
+
+
+def test_two_spaces_and_index():
+    rng = np.random.default_rng(3)
+    n = 2000
+    a = rng.integers(0, 300, size=(n, 2)).astype(np.int32)
+    b = rng.integers(0, 50, size=(n, 3)).astype(np.int32)
+    race = [(a, 2, 0, 's1', 300), (a, 2, 1, 's1', 300), (b, 3, -1, 's2', 50)]
+    p = Plan(n, 128, True, race)
+    check_plan(p, [(a, 0, 's1'), (a, 1, 's1'), (b, -1, 's2')], n)
+
+
+def test_all_elements_one_target():
+    """Every element increments one target: one element per thread colour."""
+    n = 300
+    v = np.zeros((n, 1), dtype=np.int32)
+    p = Plan(n, 64, True, [(v, 1, 0, 'u', 1)])
+    check_plan(p, [(v, 0, 'u')], n)
+    assert p.ncolors == p.nblocks
+    assert (p.nthrcol == p.nelems).all()
+
+
+@pytest.mark.parametrize('reorder', [True, False])
+def test_airfoil_edges(reorder):
+    m = meshes.QuadMesh(120, 60, seed=5)
+    race = [(m.pecell, 2, 0, 'cells', m.ncell), (m.pecell, 2, 1, 'cells', m.ncell)]
+    p = Plan(m.nedge, 256, reorder, race)
+    check_plan(p, [(m.pecell, 0, 'cells'), (m.pecell, 1, 'cells')], m.nedge)
+    if reorder:
+        # compact regions: few block colours, few thread colours
+        assert p.ncolors <= 8
+        assert p.nthrcol.max() <= 8
+    else:
+        # contiguous blocks of a permuted numbering conflict with most others
+        assert p.ncolors > 20
