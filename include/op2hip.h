@@ -125,17 +125,44 @@ int op2hip_spmv(int n, long long nnz, const int* rowptr, const int* colidx, cons
 int op2hip_csr_zero_rows(int nlist, const int* rows, const int* rowptr, const int* colidx,
                          double* vals, double diag, void* stream);
 
-/* Jacobi-preconditioned CG from x = 0, PETSc KSPCG semantics.
- * Replaces Solver.solve (pyop2/petsc_base.py:182-220; parameters
- * pyop2/base.py:1591-1601).  work: device doubles of
- * op2hip_pcg_work_size(n); reshist_host: optional max_it+1 doubles;
- * reason: PETSc KSPConvergedReason (2 rtol, 3 atol, -3 its, -4 dtol,
- * -8 indefinite, -9 nan). */
-int op2hip_pcg(int n, long long nnz, const int* rowptr, const int* colidx, const double* vals,
-               const double* b, double* x, double rtol, double atol, double dtol, int maxit,
-               double* work, double* reshist_host, int chunk, int* its_out, double* rnorm_out,
-               int* reason_out, void* stream);
-long long op2hip_pcg_work_size(int n);
+/* Batched Jacobi-preconditioned CG from x = 0, PETSc KSPCG semantics per
+ * system.  Replaces Solver.solve (pyop2/petsc_base.py:182-220; parameters
+ * pyop2/base.py:1591-1601); a single solve is a batch of one.  A system is
+ * rows [row0, row0+n) of a CSR (global column numbers) with its vectors;
+ * reason is PETSc's KSPConvergedReason (2 rtol, 3 atol, -3 its, -4 dtol,
+ * -8 indefinite, -9 nan).  Reductions are per 256-row chunk of a system,
+ * summed in chunk order, so a system's result does not depend on the other
+ * systems of the batch. */
+typedef struct op2hip_cg_system {
+  const int* rowptr;
+  const int* colidx;
+  const double* vals;
+  const double* b;
+  double* x;
+  double* inv_diag;
+  double* r;
+  double* z;
+  double* p;
+  double* w;
+  int row0;
+  int n;
+  int chunk0;
+  int nchunks;
+  double rtol, atol, dtol;
+  int maxit;
+  int state; /* 0 empty, 1 new, 2 running, 3 done */
+  double beta, betaold, dpi, dpiold, dp, rnorm0, ttol, alpha;
+  int its, reason;
+} op2hip_cg_system;
+
+int op2hip_cgb_create(int max_sys, long long max_chunks, int hist_len, void** out);
+int op2hip_cgb_destroy(void* h);
+int op2hip_cgb_load(void* h, int nsys, const op2hip_cg_system* systems, void* stream);
+int op2hip_cgb_step(void* h, int iters, void* stream);
+int op2hip_cgb_read(void* h, op2hip_cg_system* out, void* stream);
+int op2hip_cgb_history(void* h, int k, double* out, void* stream);
+int op2hip_cgb_sizeof_system(void);
+int op2hip_cgb_chunk_rows(void);
 
 #ifdef __cplusplus
 }

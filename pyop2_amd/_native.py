@@ -22,6 +22,28 @@ if not os.path.exists(LIB_PATH):
     raise ImportError("libop2hip.so is not built (expected at %s); run "
                       "`python -m pyop2_amd.build` first" % LIB_PATH)
 
+
+
+def _preload_torch_hip():
+    """Load the HIP runtime torch ships before libop2hip.so.
+
+    torch brings its own libamdhip64/libhsa-runtime64/libhiprtc (same
+    sonames as /opt/rocm's).  libop2hip.so must bind to those copies: two HIP
+    runtimes in one process each open the GPU, and the second finds none.
+    Loading torch's first makes the dynamic linker resolve libop2hip.so's
+    dependencies to them by soname."""
+    try:
+        import torch
+    except ImportError:
+        return
+    libdir = os.path.join(os.path.dirname(torch.__file__), 'lib')
+    for name in ('libhsa-runtime64.so', 'libamdhip64.so', 'libamd_comgr.so', 'libhiprtc.so'):
+        path = os.path.join(libdir, name)
+        if os.path.exists(path):
+            ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+_preload_torch_hip()
 lib = ctypes.CDLL(LIB_PATH)
 
 c_int = ctypes.c_int
@@ -66,10 +88,14 @@ _SIGS = {
     'op2hip_sparsity_discard': (c_int, [c_void]),
     'op2hip_spmv': (c_int, [c_int, c_ll, c_void, c_void, c_void, c_void, c_void, c_void]),
     'op2hip_csr_zero_rows': (c_int, [c_int, c_void, c_void, c_void, c_void, c_double, c_void]),
-    'op2hip_pcg': (c_int, [c_int, c_ll, c_void, c_void, c_void, c_void, c_void, c_double,
-                           c_double, c_double, c_int, c_void, c_void, c_int, P(c_int),
-                           P(c_double), P(c_int), c_void]),
-    'op2hip_pcg_work_size': (c_ll, [c_int]),
+    'op2hip_cgb_create': (c_int, [c_int, c_ll, c_int, P(c_void)]),
+    'op2hip_cgb_destroy': (c_int, [c_void]),
+    'op2hip_cgb_load': (c_int, [c_void, c_int, c_void, c_void]),
+    'op2hip_cgb_step': (c_int, [c_void, c_int, c_void]),
+    'op2hip_cgb_read': (c_int, [c_void, c_void, c_void]),
+    'op2hip_cgb_history': (c_int, [c_void, c_int, c_void, c_void]),
+    'op2hip_cgb_sizeof_system': (c_int, []),
+    'op2hip_cgb_chunk_rows': (c_int, []),
 }
 
 for _name, (_res, _args) in _SIGS.items():

@@ -1,0 +1,112 @@
+"""Host side of the device batched Jacobi-PCG (csrc/cg_batch.hip).
+
+A :class:`CGBatch` holds up to ``max_sys`` systems; each is a row range of
+an FP64 CSR matrix with its vectors on the device.  ``solve`` runs systems
+to completion, ``load``/``step``/``read`` let a caller (the serving loop)
+add and retire systems between iteration launches.  PETSc KSPCG semantics
+per system (reference: pyop2/petsc_base.py:172-220)."""
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+
+SYS_EMPTY, SYS_NEW, SYS_RUNNING, SYS_DONE = 0, 1, 2, 3
+
+REASONS = {2: 'CONVERGED_RTOL', 3: 'CONVERGED_ATOL', -3: 'DIVERGED_ITS', -4: 'DIVERGED_DTOL',
+           -8: 'DIVERGED_INDEFINITE_MAT', -9: 'DIVERGED_NANORINF'}
+
+_p = ctypes.c_void_p
+_d = ctypes.c_double
+_i = ctypes.c_int
+
+
+class CGSystem(ctypes.Structure):
+    """Mirror of ``op2hip_cg_system`` (include/op2hip.h)."""
+    _fields_ = [('rowptr', _p), ('colidx', _p), ('vals', _p), ('b', _p), ('x', _p),
+                ('inv_diag', _p), ('r', _p), ('z', _p), ('p', _p), ('w', _p),
+                ('row0', _i), ('n', _i), ('chunk0', _i), ('nchunks', _i),
+                ('rtol', _d), ('atol', _d), ('dtol', _d), ('maxit', _i), ('state', _i),
+                ('beta', _d), ('betaold', _d), ('dpi', _d), ('dpiold', _d), ('dp', _d),
+                ('rnorm0', _d), ('ttol', _d), ('alpha', _d), ('its', _i), ('reason', _i)]
+
+
+CHUNK = None
+
+
+def chunk_rows():
+    global CHUNK
+    if CHUNK is None:
+        CHUNK = nat.lib.op2hip_cgb_chunk_rows()
+    return CHUNK
+
+
+def make_system(rowptr, colidx, vals, b, x, work, row0, n, rtol, atol, dtol, maxit):
+    """A NEW system over rows [row0, row0+n).  ``work`` holds 5 device
+    vectors (inv_diag, r, z, p, w) laid out like x (indexed by global row):
+    pass the base addresses of 5 arrays of the global length."""
+    s = CGSystem()
+    s.rowptr, s.colidx, s.vals, s.b, s.x = rowptr, colidx, vals, b, x
+    s.inv_diag, s.r, s.z, s.p, s.w = work
+    s.row0, s.n = row0, n
+    s.rtol, s.atol, s.dtol, s.maxit = rtol, atol, dtol, maxit
+    s.state = SYS_NEW
+    return s
+
+
+class CGBatch(object):
+
+    def __init__(self, max_sys, max_rows, hist_len=0):
+        self.max_sys = max_sys
+        ch = chunk_rows()
+        self.max_chunks = max(1, (max_rows + ch - 1) // ch + max_sys)
+        h = ctypes.c_void_p()
+        nat.check(nat.lib.op2hip_cgb_create(max_sys, self.max_chunks, hist_len, ctypes.byref(h)),
+                  'cg batch create')
+        self._h = h
+        self.hist_len = hist_len
+        self.systems = (CGSystem * max_sys)()
+        self.nsys = 0
+
+    def load(self, systems, stream):
+        n = len(systems)
+        for k, s in enumerate(systems):
+            self.systems[k] = s
+        self.nsys = n
+        nat.check(nat.lib.op2hip_cgb_load(self._h, n, ctypes.byref(self.systems),
+                                          ctypes.c_void_p(stream)), 'cg batch load')
+
+    def step(self, iters, stream):
+        nat.check(nat.lib.op2hip_cgb_step(self._h, iters, ctypes.c_void_p(stream)), 'cg step')
+
+    def read(self, stream):
+        nat.check(nat.lib.op2hip_cgb_read(self._h, ctypes.byref(self.systems),
+                                          ctypes.c_void_p(stream)), 'cg batch read')
+        return [self.systems[k] for k in range(self.nsys)]
+
+    def history(self, k, stream):
+        out = np.zeros(self.hist_len)
+        nat.check(nat.lib.op2hip_cgb_history(self._h, k, out.ctypes.data_as(ctypes.c_void_p),
+                                             ctypes.c_void_p(stream)), 'cg history')
+        return out
+
+    def solve(self, systems, stream, chunk=16):
+        """Run ``systems`` (all NEW) to completion; returns their final
+        descriptors."""
+        self.load(systems, stream)
+        maxit = max(s.maxit for s in systems) if systems else 0
+        done_iters = 0
+        while True:
+            k = chunk if done_iters else min(chunk, 4)
+            self.step(k, stream)
+            done_iters += k
+            out = self.read(stream)
+            if all(s.state == SYS_DONE for s in out) or done_iters > maxit + chunk:
+                return out
+
+    def __del__(self):
+        try:
+            nat.lib.op2hip_cgb_destroy(self._h)
+        except Exception:
+            pass

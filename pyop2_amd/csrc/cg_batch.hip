@@ -1,0 +1,436 @@
+// Batched Jacobi-preconditioned CG: many independent FP64 CSR systems advanced
+// together, each with its own scalars, iteration count and convergence test.
+//
+// Semantics per system are PETSc's KSPSolve_CG with PCJACOBI and the
+// preconditioned norm (pyop2/petsc_base.py:172-220; defaults
+// pyop2/base.py:1591-1601), as in the single solve:
+//   r = b, z = D^-1 r, dp = |z|, test at it 0, beta = z.r
+//   loop: p = z (+ beta/betaold p); w = A p; dpi = p.w; a = beta/dpi;
+//         x += a p; r -= a w; z = D^-1 r; dp = |z|; test; beta = z.r
+//   ttol = max(rtol*|z0|, atol); converged dp <= ttol; diverged
+//   dp >= dtol*|z0|; DIVERGED_ITS after max_it.
+//
+// Work is cut into chunks of kChunk rows counted from each system's first
+// row; every reduction is per-chunk partials summed in chunk order by one
+// workgroup per system.  A system's arithmetic therefore does not depend on
+// which other systems share the launch: a solve gives bit-identical results
+// alone and in a batch.  Systems join (state NEW) and leave (DONE) between
+// iteration launches, which is what the serving loop's continuous batching
+// uses; kernels skip chunks of systems in other states.
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kChunk = 256;   // rows per chunk (one row per thread)
+constexpr int kLanes = 8;     // lanes per row in SpMV
+
+enum { SYS_EMPTY = 0, SYS_NEW = 1, SYS_RUNNING = 2, SYS_DONE = 3 };
+
+enum {
+  KSP_CONVERGED_RTOL = 2,
+  KSP_CONVERGED_ATOL = 3,
+  KSP_DIVERGED_ITS = -3,
+  KSP_DIVERGED_DTOL = -4,
+  KSP_DIVERGED_INDEFINITE_MAT = -8,
+  KSP_DIVERGED_NANORINF = -9,
+};
+
+}  // namespace
+
+extern "C" {
+
+// One system.  Vectors are indexed by global row i in [row0, row0 + n);
+// colidx holds global column numbers into the same vectors.
+struct op2hip_cg_system {
+  const int* rowptr;
+  const int* colidx;
+  const double* vals;
+  const double* b;
+  double* x;
+  double* inv_diag;
+  double* r;
+  double* z;
+  double* p;
+  double* w;
+  int row0;
+  int n;
+  int chunk0;    // first entry of this system in the chunk list
+  int nchunks;
+  double rtol, atol, dtol;
+  int maxit;
+  int state;
+  // solver scalars
+  double beta, betaold, dpi, dpiold, dp, rnorm0, ttol, alpha;
+  int its, reason;
+};
+
+}  // extern "C"
+
+namespace {
+
+using Sys = op2hip_cg_system;
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kBlock / 64; ++k) t += sh[k];
+  __syncthreads();
+  return t;
+}
+
+// chunk list entry: (system, chunk index within the system)
+struct ChunkRef {
+  int sys, chunk;
+};
+
+__device__ inline void chunk_rows(const Sys& s, int c, int& lo, int& hi) {
+  lo = s.row0 + c * kChunk;
+  hi = lo + kChunk;
+  if (hi > s.row0 + s.n) hi = s.row0 + s.n;
+}
+
+// NEW systems: Jacobi diagonal, x = 0, r = b, z = D^-1 r, partials z.z, z.r
+__global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunks,
+                                                 double* part) {
+  __shared__ double sh[kBlock / 64];
+  const ChunkRef cr = chunks[blockIdx.x];
+  Sys& s = sys[cr.sys];
+  if (s.state != SYS_NEW) return;
+  int lo, hi;
+  chunk_rows(s, cr.chunk, lo, hi);
+  double zz = 0.0, zr = 0.0;
+  const int i = lo + threadIdx.x;
+  if (i < hi) {
+    double d = 0.0;
+    for (int k = s.rowptr[i]; k < s.rowptr[i + 1]; ++k)
+      if (s.colidx[k] == i) d = s.vals[k];
+    double inv = d != 0.0 ? 1.0 / d : 1.0;
+    s.inv_diag[i] = inv;
+    double ri = s.b[i];
+    double zi = inv * ri;
+    s.x[i] = 0.0;
+    s.r[i] = ri;
+    s.z[i] = zi;
+    zz = zi * zi;
+    zr = zi * ri;
+  }
+  double t = block_sum(zz, sh);
+  double u = block_sum(zr, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t;
+    part[2 * blockIdx.x + 1] = u;
+  }
+}
+
+__device__ inline void reduce_pair(const Sys& s, const double* part, double* sh, double& a,
+                                   double& b) {
+  double va = 0.0, vb = 0.0;
+  for (int c = threadIdx.x; c < s.nchunks; c += kBlock) {
+    va += part[2 * (s.chunk0 + c)];
+    vb += part[2 * (s.chunk0 + c) + 1];
+  }
+  a = block_sum(va, sh);
+  b = block_sum(vb, sh);
+}
+
+__device__ inline void check(Sys& s) {
+  double dp = s.dp;
+  if (!(dp == dp) || isinf(dp)) {
+    s.reason = KSP_DIVERGED_NANORINF;
+    s.state = SYS_DONE;
+  } else if (dp <= s.ttol) {
+    s.reason = dp < s.atol ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
+    s.state = SYS_DONE;
+  } else if (dp >= s.dtol * s.rnorm0) {
+    s.reason = KSP_DIVERGED_DTOL;
+    s.state = SYS_DONE;
+  }
+}
+
+// one workgroup per system
+__global__ void __launch_bounds__(kBlock) k_start(Sys* sys, const double* part,
+                                                  double* reshist, int hist_stride) {
+  __shared__ double sh[kBlock / 64];
+  Sys& s = sys[blockIdx.x];
+  if (s.state != SYS_NEW) return;
+  double zz, zr;
+  reduce_pair(s, part, sh, zz, zr);
+  if (threadIdx.x == 0) {
+    s.dp = sqrt(zz);
+    s.rnorm0 = s.dp;
+    s.ttol = fmax(s.rtol * s.rnorm0, s.atol);
+    s.its = 0;
+    s.reason = 0;
+    s.dpi = 0.0;
+    s.state = SYS_RUNNING;
+    if (reshist) reshist[(long long)blockIdx.x * hist_stride] = s.dp;
+    check(s);
+    s.beta = zr;
+    if (s.state == SYS_RUNNING && s.maxit <= 0) {
+      s.reason = KSP_DIVERGED_ITS;
+      s.state = SYS_DONE;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_update_p(Sys* sys, const ChunkRef* chunks) {
+  const ChunkRef cr = chunks[blockIdx.x];
+  Sys& s = sys[cr.sys];
+  if (s.state != SYS_RUNNING) return;
+  int lo, hi;
+  chunk_rows(s, cr.chunk, lo, hi);
+  const int i = lo + threadIdx.x;
+  if (i >= hi) return;
+  if (s.its == 0) {
+    s.p[i] = s.z[i];
+  } else {
+    double bb = s.beta / s.betaold;
+    s.p[i] = s.z[i] + bb * s.p[i];
+  }
+}
+
+// w = A p for the chunk's rows, partial p.w
+__global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunks,
+                                                 double* part) {
+  __shared__ double sh[kBlock / 64];
+  const ChunkRef cr = chunks[blockIdx.x];
+  Sys& s = sys[cr.sys];
+  if (s.state != SYS_RUNNING) return;
+  int lo, hi;
+  chunk_rows(s, cr.chunk, lo, hi);
+  const int sub = threadIdx.x & (kLanes - 1);
+  double acc = 0.0;
+  for (int i = lo + (int)threadIdx.x / kLanes; i < hi; i += kBlock / kLanes) {
+    const int kb = s.rowptr[i], ke = s.rowptr[i + 1];
+    double v = 0.0;
+    for (int k = kb + sub; k < ke; k += kLanes) v += s.vals[k] * s.p[s.colidx[k]];
+    for (int off = kLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kLanes);
+    if (sub == 0) {
+      s.w[i] = v;
+      acc += s.p[i] * v;
+    }
+  }
+  double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kBlock) k_alpha(Sys* sys, const double* part) {
+  __shared__ double sh[kBlock / 64];
+  Sys& s = sys[blockIdx.x];
+  if (s.state != SYS_RUNNING) return;
+  double v = 0.0;
+  for (int c = threadIdx.x; c < s.nchunks; c += kBlock) v += part[2 * (s.chunk0 + c)];
+  double pw = block_sum(v, sh);
+  if (threadIdx.x == 0) {
+    s.its += 1;
+    if (s.beta == 0.0) {
+      s.reason = KSP_CONVERGED_ATOL;
+      s.state = SYS_DONE;
+      return;
+    }
+    s.dpiold = s.dpi;
+    s.dpi = pw;
+    if (pw == 0.0 || (s.its > 1 && s.dpi * s.dpiold <= 0.0)) {
+      s.reason = KSP_DIVERGED_INDEFINITE_MAT;
+      s.state = SYS_DONE;
+      return;
+    }
+    s.alpha = s.beta / pw;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_axpy(Sys* sys, const ChunkRef* chunks, double* part) {
+  __shared__ double sh[kBlock / 64];
+  const ChunkRef cr = chunks[blockIdx.x];
+  Sys& s = sys[cr.sys];
+  if (s.state != SYS_RUNNING) return;
+  int lo, hi;
+  chunk_rows(s, cr.chunk, lo, hi);
+  const int i = lo + threadIdx.x;
+  double zz = 0.0, zr = 0.0;
+  if (i < hi) {
+    const double a = s.alpha;
+    s.x[i] = s.x[i] + a * s.p[i];
+    double ri = s.r[i] - a * s.w[i];
+    s.r[i] = ri;
+    double zi = s.inv_diag[i] * ri;
+    s.z[i] = zi;
+    zz = zi * zi;
+    zr = zi * ri;
+  }
+  double t = block_sum(zz, sh);
+  double u = block_sum(zr, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t;
+    part[2 * blockIdx.x + 1] = u;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_finish(Sys* sys, const double* part, double* reshist,
+                                                   int hist_stride) {
+  __shared__ double sh[kBlock / 64];
+  Sys& s = sys[blockIdx.x];
+  if (s.state != SYS_RUNNING) return;
+  double zz, zr;
+  reduce_pair(s, part, sh, zz, zr);
+  if (threadIdx.x == 0) {
+    s.dp = sqrt(zz);
+    if (reshist && s.its < hist_stride)
+      reshist[(long long)blockIdx.x * hist_stride + s.its] = s.dp;
+    check(s);
+    if (s.state == SYS_RUNNING && s.its >= s.maxit) {
+      s.reason = KSP_DIVERGED_ITS;
+      s.state = SYS_DONE;
+    }
+    if (s.state == SYS_RUNNING) {
+      s.betaold = s.beta;
+      s.beta = zr;
+    }
+  }
+}
+
+struct Batch {
+  int max_sys = 0;
+  long long max_chunks = 0;
+  int nsys = 0;
+  long long nchunks = 0;
+  Sys* d_sys = nullptr;
+  ChunkRef* d_chunks = nullptr;
+  double* d_part = nullptr;
+  double* d_hist = nullptr;
+  int hist_stride = 0;
+  std::vector<Sys> h_sys;
+};
+
+}  // namespace
+
+extern "C" {
+
+// A batch of up to max_sys systems and max_chunks chunks of 256 rows;
+// hist_len > 0 keeps a residual history of hist_len entries per system.
+int op2hip_cgb_create(int max_sys, long long max_chunks, int hist_len, void** out) {
+  OP2HIP_REQUIRE(max_sys > 0 && max_chunks > 0 && out, "bad batch sizes");
+  Batch* b = new Batch();
+  b->max_sys = max_sys;
+  b->max_chunks = max_chunks;
+  b->hist_stride = hist_len;
+  hipError_t e = hipMalloc(&b->d_sys, sizeof(Sys) * max_sys);
+  if (e == hipSuccess) e = hipMalloc(&b->d_chunks, sizeof(ChunkRef) * max_chunks);
+  if (e == hipSuccess) e = hipMalloc(&b->d_part, sizeof(double) * 2 * max_chunks);
+  if (e == hipSuccess && hist_len > 0)
+    e = hipMalloc(&b->d_hist, sizeof(double) * (size_t)hist_len * max_sys);
+  if (e != hipSuccess) {
+    if (b->d_sys) hipFree(b->d_sys);
+    if (b->d_chunks) hipFree(b->d_chunks);
+    if (b->d_part) hipFree(b->d_part);
+    delete b;
+    op2hip::set_error("batch allocation failed: %s", hipGetErrorString(e));
+    return OP2HIP_ERR_HIP;
+  }
+  *out = b;
+  return OP2HIP_OK;
+}
+
+int op2hip_cgb_destroy(void* h) {
+  Batch* b = static_cast<Batch*>(h);
+  if (!b) return OP2HIP_OK;
+  hipFree(b->d_sys);
+  hipFree(b->d_chunks);
+  hipFree(b->d_part);
+  if (b->d_hist) hipFree(b->d_hist);
+  delete b;
+  return OP2HIP_OK;
+}
+
+// Load the batch's systems (host array of nsys descriptors; state NEW for
+// systems to start, RUNNING/DONE to keep, EMPTY for unused slots) and
+// rebuild the chunk list.  Solver scalars of RUNNING/DONE systems are taken
+// from the descriptors, so a caller that keeps systems across calls passes
+// back what op2hip_cgb_read returned.
+int op2hip_cgb_load(void* h, int nsys, const op2hip_cg_system* systems, void* stream) {
+  Batch* b = static_cast<Batch*>(h);
+  OP2HIP_REQUIRE(b && nsys >= 0 && nsys <= b->max_sys, "bad batch load (%d systems)", nsys);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  b->h_sys.assign(systems, systems + nsys);
+  std::vector<ChunkRef> chunks;
+  for (int k = 0; k < nsys; ++k) {
+    Sys& sy = b->h_sys[k];
+    sy.chunk0 = static_cast<int>(chunks.size());
+    sy.nchunks = sy.state == SYS_EMPTY ? 0 : (sy.n + kChunk - 1) / kChunk;
+    for (int c = 0; c < sy.nchunks; ++c) chunks.push_back({k, c});
+  }
+  OP2HIP_REQUIRE(static_cast<long long>(chunks.size()) <= b->max_chunks,
+                 "batch needs %zu chunks, capacity %lld", chunks.size(), b->max_chunks);
+  b->nsys = nsys;
+  b->nchunks = static_cast<long long>(chunks.size());
+  if (nsys) OP2HIP_CHECK(hipMemcpyAsync(b->d_sys, b->h_sys.data(), sizeof(Sys) * nsys,
+                                        hipMemcpyHostToDevice, s));
+  if (!chunks.empty())
+    OP2HIP_CHECK(hipMemcpyAsync(b->d_chunks, chunks.data(), sizeof(ChunkRef) * chunks.size(),
+                                hipMemcpyHostToDevice, s));
+  OP2HIP_CHECK(hipStreamSynchronize(s));
+  return OP2HIP_OK;
+}
+
+// Start NEW systems and run `iters` iterations of every RUNNING one.
+// Enqueues only (no host synchronisation).
+int op2hip_cgb_step(void* h, int iters, void* stream) {
+  Batch* b = static_cast<Batch*>(h);
+  OP2HIP_REQUIRE(b, "null batch");
+  if (b->nsys == 0 || b->nchunks == 0) return OP2HIP_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dim3 gc(static_cast<unsigned>(b->nchunks)), gs(b->nsys), blk(kBlock);
+  hipLaunchKernelGGL(k_init, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
+  hipLaunchKernelGGL(k_start, gs, blk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_update_p, gc, blk, 0, s, b->d_sys, b->d_chunks);
+    hipLaunchKernelGGL(k_spmv, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
+    hipLaunchKernelGGL(k_alpha, gs, blk, 0, s, b->d_sys, b->d_part);
+    hipLaunchKernelGGL(k_axpy, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
+    hipLaunchKernelGGL(k_finish, gs, blk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
+  }
+  OP2HIP_CHECK(hipGetLastError());
+  return OP2HIP_OK;
+}
+
+// Copy the systems' descriptors (with state, its, reason, dp) back to the
+// host; waits for the stream.
+int op2hip_cgb_read(void* h, op2hip_cg_system* out, void* stream) {
+  Batch* b = static_cast<Batch*>(h);
+  OP2HIP_REQUIRE(b && (out || b->nsys == 0), "bad batch read");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (b->nsys)
+    OP2HIP_CHECK(hipMemcpyAsync(out, b->d_sys, sizeof(Sys) * b->nsys, hipMemcpyDeviceToHost, s));
+  OP2HIP_CHECK(hipStreamSynchronize(s));
+  return OP2HIP_OK;
+}
+
+// Residual history of system k (hist_len entries) into host `out`.
+int op2hip_cgb_history(void* h, int k, double* out, void* stream) {
+  Batch* b = static_cast<Batch*>(h);
+  OP2HIP_REQUIRE(b && b->d_hist && k >= 0 && k < b->nsys, "no history for system %d", k);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  OP2HIP_CHECK(hipMemcpyAsync(out, b->d_hist + (size_t)k * b->hist_stride,
+                              sizeof(double) * b->hist_stride, hipMemcpyDeviceToHost, s));
+  OP2HIP_CHECK(hipStreamSynchronize(s));
+  return OP2HIP_OK;
+}
+
+int op2hip_cgb_sizeof_system(void) { return static_cast<int>(sizeof(Sys)); }
+
+int op2hip_cgb_chunk_rows(void) { return kChunk; }
+
+}  // extern "C"

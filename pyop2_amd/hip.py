@@ -30,6 +30,7 @@ from . import base
 from . import configuration as cfg
 from . import _native as nat
 from . import jit
+from . import cg
 from .base import (READ, WRITE, RW, INC, MIN, MAX, IdentityMap, IterationIndex, i,
                    IterationSpace, Halo, Access)
 from .codegen import LoopSource
@@ -285,9 +286,6 @@ class Solver(base.Solver):
     """Jacobi-preconditioned CG on the device with PETSc KSPCG semantics
     (reference: petsc_base.py:160-220)."""
 
-    REASONS = {2: 'CONVERGED_RTOL', 3: 'CONVERGED_ATOL', -3: 'DIVERGED_ITS',
-               -4: 'DIVERGED_DTOL', -8: 'DIVERGED_INDEFINITE_MAT', -9: 'DIVERGED_NANORINF'}
-
     def solve(self, A, x, b):
         p = self.parameters
         if p['linear_solver'] != 'cg':
@@ -302,34 +300,29 @@ class Solver(base.Solver):
         sp = A.sparsity
         n = sp.nrows * A.dims[0]
         bdev = b._to_device()
-        x._to_device()
-        work = torch.empty(int(nat.lib.op2hip_pcg_work_size(n)), dtype=torch.float64,
-                           device=device())
+        xdev = x._to_device()
+        work = torch.empty((5, max(n, 1)), dtype=torch.float64, device=device())
         maxit = int(p['maximum_iterations'])
-        hist = np.zeros(maxit + 1) if p['monitor_convergence'] else None
-        its = ctypes.c_int()
-        rnorm = ctypes.c_double()
-        reason = ctypes.c_int()
-        nat.check(nat.lib.op2hip_pcg(
-            n, sp.nz, ctypes.c_void_p(_ptr(sp._dev_rowptr)), ctypes.c_void_p(_ptr(sp._dev_colidx)),
-            ctypes.c_void_p(_ptr(A._dev_vals)), ctypes.c_void_p(_ptr(bdev)),
-            ctypes.c_void_p(_ptr(x._dev)), float(p['relative_tolerance']),
-            float(p['absolute_tolerance']), float(p['divergence_tolerance']), maxit,
-            ctypes.c_void_p(_ptr(work)),
-            hist.ctypes.data_as(ctypes.c_void_p) if hist is not None else None,
-            int(p.get('hip_cg_chunk', 16)), ctypes.byref(its), ctypes.byref(rnorm),
-            ctypes.byref(reason), ctypes.c_void_p(stream())), 'pcg')
+        s = stream()
+        sysd = cg.make_system(_ptr(sp._dev_rowptr), _ptr(sp._dev_colidx), _ptr(A._dev_vals),
+                              _ptr(bdev), _ptr(xdev), [_ptr(work[k]) for k in range(5)], 0, n,
+                              float(p['relative_tolerance']), float(p['absolute_tolerance']),
+                              float(p['divergence_tolerance']), maxit)
+        hist_len = maxit + 1 if p['monitor_convergence'] else 0
+        batch = cg.CGBatch(1, n, hist_len)
+        out = batch.solve([sysd], s, chunk=int(p.get('hip_cg_chunk', 16)))[0]
+        its, rnorm, reason = out.its, out.dp, out.reason
+        if hist_len:
+            self.reshist = list(batch.history(0, s)[:its + 1])
         x._device_written()
         x.needs_halo_update = True
-        self._its, self._rnorm, self._reason = its.value, rnorm.value, reason.value
-        if hist is not None:
-            self.reshist = list(hist[:its.value + 1])
-        debug("Converged reason: %s" % self.REASONS.get(reason.value, reason.value))
-        debug("Iterations: %s" % its.value)
-        debug("Residual norm: %s" % rnorm.value)
-        if reason.value < 0:
+        self._its, self._rnorm, self._reason = its, rnorm, reason
+        debug("Converged reason: %s" % cg.REASONS.get(reason, reason))
+        debug("Iterations: %s" % its)
+        debug("Residual norm: %s" % rnorm)
+        if reason < 0:
             msg = "KSP Solver failed to converge in %d iterations: %s (Residual norm: %e)" \
-                % (its.value, self.REASONS.get(reason.value, reason.value), rnorm.value)
+                % (its, cg.REASONS.get(reason, reason), rnorm)
             if p['error_on_nonconvergence']:
                 raise RuntimeError(msg)
             warning(msg)
@@ -476,15 +469,6 @@ class _CompiledLoop(object):
 
 
 _loop_cache = {}
-_scratch = {}
-
-
-def _partials(pos, nbytes):
-    t = _scratch.get(pos)
-    if t is None or t.numel() < nbytes:
-        t = _dev_bytes(max(nbytes, 4096))
-        _scratch[pos] = t
-    return t
 
 
 class JITModule(base.JITModule):
@@ -497,6 +481,163 @@ class JITModule(base.JITModule):
 
 def _block_size():
     return cfg.get('hip_block_size') or 256
+
+
+class _LaunchEntry(object):
+    """Everything one recurring ``par_loop`` call needs, resolved once: the
+    compiled loop, device pointers packed into ctypes argument arrays per
+    launch, reduction scratch, and the coherence updates to apply.  Holds
+    strong references to the loop's data objects, so the pointers stay
+    valid for as long as the entry is cached."""
+
+    __slots__ = ('loop', 'launches', 'to_device', 'written', 'dats', 'globals_red', 'mats',
+                 'consts', 'keep')
+
+    def __init__(self, pl, mode, bs):
+        consts = base.Const._definitions()
+        self.consts = consts
+        loop = pl._compiled(mode, bs, consts)
+        self.loop = loop
+        args = pl.args
+        n = pl._it_space.exec_size if pl.needs_exec_halo else pl._it_space.size
+        self.keep = [pl.kernel, args, pl._it_space.iterset]
+        self.to_device = []
+        for a in args:
+            if (a._is_dat or a._is_global) and a.data not in self.to_device:
+                self.to_device.append(a.data)
+        for d in self.to_device:
+            d._to_device()
+        plan = None
+        if mode == 'coloured':
+            plan = get_plan(pl._it_space.iterset, n, args,
+                            part_size=cfg.get('hip_partition_size') or bs)
+            nparts = plan.nblocks
+            grid_stream = 0
+        else:
+            grid_stream = max(1, min((n + bs - 1) // bs, 4096))
+            nparts = grid_stream
+        partials = {}
+        vals = []
+        for role, pos in loop.source.param_roles():
+            a = args[pos] if pos is not None else None
+            if role in ('start', 'count') or role.startswith('plan_'):
+                vals.append(role)
+            elif role == 'gbl':
+                vals.append(_ptr(a.data._dev))
+            elif role == 'partial':
+                t = _dev_bytes(max(nparts, 1) * a.data.cdim * a.data.dtype.itemsize)
+                partials[pos] = t
+                vals.append(_ptr(t))
+            elif role == 'data':
+                vals.append(_ptr(a.data._dev))
+            elif role == 'map':
+                vals.append(_ptr(a.map._device_values()))
+            elif role == 'mat_vals':
+                vals.append(_ptr(a.data._dev_vals))
+            elif role == 'mat_rowptr':
+                vals.append(_ptr(a.data.sparsity._dev_rowptr))
+            elif role == 'mat_colidx':
+                vals.append(_ptr(a.data.sparsity._dev_colidx))
+            elif role == 'mat_rmap':
+                vals.append(_ptr(a.map[0]._device_values()))
+            elif role == 'mat_cmap':
+                vals.append(_ptr(a.map[1]._device_values()))
+            elif role == 'err_flag':
+                vals.append(_ptr([x for x in args if x._is_mat][0].data._err))
+            else:
+                raise RuntimeError("unknown parameter role %s" % role)
+        self.keep.append(partials)
+        launches = []
+        if n > 0:
+            if mode == 'stream':
+                argv = [0 if r == 'start' else n if r == 'count' else r for r in vals]
+                launches.append((loop.fn, grid_stream, bs, _argv(argv)))
+            else:
+                pdev = plan.device_arrays()
+                self.keep.append(plan)
+                names = {'plan_elems': 'elems', 'plan_blkmap': 'blkmap',
+                         'plan_offset': 'offset', 'plan_nelem': 'nelems',
+                         'plan_nthrcol': 'nthrcol', 'plan_thrcol': 'thrcol'}
+                for col in range(plan.ncolors):
+                    cnt = int(plan.ncolblk[col])
+                    if cnt == 0:
+                        continue
+                    argv = []
+                    for r in vals:
+                        if r == 'start':
+                            argv.append(int(plan.colour_start[col]))
+                        elif r == 'count':
+                            argv.append(cnt)
+                        elif isinstance(r, str):
+                            argv.append(_ptr(pdev[names[r]]))
+                        else:
+                            argv.append(r)
+                    launches.append((loop.fn, cnt, bs, _argv(argv)))
+            if loop.reduce_fn is not None:
+                rargv = []
+                for role, pos in loop.source.reduce_roles():
+                    if role == 'nparts':
+                        rargv.append(nparts)
+                    elif role == 'gbl_out':
+                        rargv.append(_ptr(args[pos].data._dev))
+                    else:
+                        rargv.append(_ptr(partials[pos]))
+                launches.append((loop.reduce_fn, 1, 256, _argv(rargv)))
+        self.launches = launches
+        self.written = [a.data for a in args if (a._is_dat and a.access is not READ) or
+                        a._is_global_reduction]
+        self.dats = [a.data for a in args if a._is_dat]
+        self.mats = [a.data for a in args if a._is_mat]
+
+    def run(self):
+        s = stream()
+        loop = self.loop
+        for c in self.consts:
+            if loop.const_versions.get(c.name) != (id(c), c._version):
+                ptr, nbytes = loop.module.global_symbol(c.name)
+                data = np.ascontiguousarray(c.data)
+                nat.check(nat.lib.op2hip_set_global(ptr, data.ctypes.data_as(ctypes.c_void_p),
+                                                    min(nbytes, data.nbytes), ctypes.c_void_p(s)),
+                          'const upload')
+                loop.const_versions[c.name] = (id(c), c._version)
+        for d in self.to_device:
+            if d._state == HOST:
+                d._to_device()
+        sp = ctypes.c_void_p(s)
+        launch = nat.lib.op2hip_launch
+        for fn, grid, bs, (arr, n) in self.launches:
+            st = launch(fn, grid, bs, 0, sp, arr, n)
+            if st != 0:
+                nat.check(st, 'kernel launch')
+        for d in self.written:
+            d._state = DEVICE
+        for d in self.dats:
+            maybe_setflags(d._data, write=False)
+
+
+def _argv(values):
+    n = len(values)
+    return ((ctypes.c_uint64 * n)(*[int(v) & 0xFFFFFFFFFFFFFFFF for v in values]), n)
+
+
+_call_cache = {}
+_CALL_CACHE_MAX = 512
+
+
+def _arg_key(a):
+    m = a.map
+    if isinstance(m, (list, tuple)):
+        mk = tuple(id(x) for x in m)
+    else:
+        mk = id(m)
+    idx = a.idx
+    if isinstance(idx, (list, tuple)):
+        ik = tuple(getattr(x, 'index', x) for x in idx)
+    elif isinstance(idx, IterationIndex):
+        ik = ('i', idx.index)
+    else:
+        ik = idx
+    return (id(a.data), mk, ik, id(a.access))
 
 
 class ParLoop(base.ParLoop):
@@ -516,119 +657,19 @@ class ParLoop(base.ParLoop):
         return c
 
     def compute(self):
-        n = self._it_space.exec_size if self.needs_exec_halo else self._it_space.size
-        mode = self._mode()
         bs = _block_size()
-        consts = base.Const._definitions()
-        loop = self._compiled(mode, bs, consts)
-        s = stream()
-        # constants
-        for c in consts:
-            if loop.const_versions.get(c.name) != (id(c), c._version):
-                ptr, nbytes = loop.module.global_symbol(c.name)
-                data = np.ascontiguousarray(c.data)
-                nat.check(nat.lib.op2hip_set_global(ptr, data.ctypes.data_as(ctypes.c_void_p),
-                                                    min(nbytes, data.nbytes), ctypes.c_void_p(s)),
-                          'const upload')
-                loop.const_versions[c.name] = (id(c), c._version)
-        # arguments to the device
-        for a in self.args:
-            if a._is_dat or a._is_global:
-                a.data._to_device()
-        plan = None
-        if mode == 'coloured':
-            plan = get_plan(self._it_space.iterset, n, self.args,
-                            part_size=cfg.get('hip_partition_size') or bs)
-            nparts = plan.nblocks
-            grid_stream = 0
-        else:
-            grid_stream = max(1, min((n + bs - 1) // bs, 4096))
-            nparts = grid_stream
-        partials = {}
-        argv_args = []
-        for role, pos in loop.source.param_roles():
-            a = self.args[pos] if pos is not None else None
-            if role in ('start', 'count') or role.startswith('plan_'):
-                argv_args.append(role)
-            elif role == 'gbl':
-                argv_args.append(_ptr(a.data._dev))
-            elif role == 'partial':
-                t = _partials(pos, nparts * a.data.cdim * a.data.dtype.itemsize)
-                partials[pos] = t
-                argv_args.append(_ptr(t))
-            elif role == 'data':
-                argv_args.append(_ptr(a.data._dev))
-            elif role == 'map':
-                argv_args.append(_ptr(a.map._device_values()))
-            elif role == 'mat_vals':
-                argv_args.append(_ptr(a.data._dev_vals))
-            elif role == 'mat_rowptr':
-                argv_args.append(_ptr(a.data.sparsity._dev_rowptr))
-            elif role == 'mat_colidx':
-                argv_args.append(_ptr(a.data.sparsity._dev_colidx))
-            elif role == 'mat_rmap':
-                argv_args.append(_ptr(a.map[0]._device_values()))
-            elif role == 'mat_cmap':
-                argv_args.append(_ptr(a.map[1]._device_values()))
-            elif role == 'err_flag':
-                mats = [x for x in self.args if x._is_mat]
-                argv_args.append(_ptr(mats[0].data._err))
-            else:
-                raise RuntimeError("unknown parameter role %s" % role)
-        if n > 0:
-            if mode == 'stream':
-                argv = [0 if r == 'start' else n if r == 'count' else r for r in argv_args]
-                jit.launch(loop.fn, grid_stream, bs, s, argv)
-            else:
-                pdev = plan.device_arrays()
-                for col in range(plan.ncolors):
-                    cnt = int(plan.ncolblk[col])
-                    if cnt == 0:
-                        continue
-                    argv = []
-                    for r in argv_args:
-                        if r == 'start':
-                            argv.append(int(plan.colour_start[col]))
-                        elif r == 'count':
-                            argv.append(cnt)
-                        elif r == 'plan_elems':
-                            argv.append(_ptr(pdev['elems']))
-                        elif r == 'plan_blkmap':
-                            argv.append(_ptr(pdev['blkmap']))
-                        elif r == 'plan_offset':
-                            argv.append(_ptr(pdev['offset']))
-                        elif r == 'plan_nelem':
-                            argv.append(_ptr(pdev['nelems']))
-                        elif r == 'plan_nthrcol':
-                            argv.append(_ptr(pdev['nthrcol']))
-                        elif r == 'plan_thrcol':
-                            argv.append(_ptr(pdev['thrcol']))
-                        else:
-                            argv.append(r)
-                    jit.launch(loop.fn, cnt, bs, s, argv)
-            if loop.reduce_fn is not None:
-                rargv = []
-                for role, pos in loop.source.reduce_roles():
-                    if role == 'nparts':
-                        rargv.append(nparts)
-                    elif role == 'gbl_out':
-                        rargv.append(_ptr(self.args[pos].data._dev))
-                    else:
-                        rargv.append(_ptr(partials[pos]))
-                jit.launch(loop.reduce_fn, 1, 256, s, rargv)
-        # coherence: written arguments are newer on the device; stale host
-        # views become read-only (reference: sequential.py:85-86)
-        for a in self.args:
-            if a._is_dat:
-                if a.access is not READ:
-                    a.data._device_written()
-                maybe_setflags(a.data._data, write=False)
-            elif a._is_global_reduction:
-                a.data._device_written()
+        key = (id(self.kernel), self._it_space.extents, id(self._it_space.iterset), bs,
+               cfg.get('hip_partition_size'), cfg.get('hip_reorder', True),
+               tuple(_arg_key(a) for a in self.args),
+               tuple((c.name, id(c)) for c in base.Const._definitions()))
+        ent = _call_cache.get(key)
+        if ent is None:
+            if len(_call_cache) >= _CALL_CACHE_MAX:
+                _call_cache.pop(next(iter(_call_cache)))
+            ent = _LaunchEntry(self, self._mode(), bs)
+            _call_cache[key] = ent
+        ent.run()
         self.maybe_set_halo_update_needed()
-        for a in self.args:
-            if a._is_mat:
-                a.data._assemble()
 
 
 def par_loop(kernel, it_space, *args):

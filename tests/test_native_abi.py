@@ -18,7 +18,7 @@ def declared():
 
 def test_header_declares_entry_points():
     names = declared()
-    assert 'op2hip_launch' in names and 'op2hip_pcg' in names
+    assert 'op2hip_launch' in names and 'op2hip_cgb_step' in names
     assert len(names) >= 25
 
 
@@ -39,3 +39,9 @@ def test_compile_needs_no_gpu():
     from pyop2_amd import jit
     code = jit.compile_source('extern "C" __global__ void k(double* x) { x[threadIdx.x] = 1.0; }')
     assert len(code) > 1000
+
+
+def test_cg_system_layout_matches_library():
+    from pyop2_amd import _native, cg
+    import ctypes as C
+    assert C.sizeof(cg.CGSystem) == _native.lib.op2hip_cgb_sizeof_system()
