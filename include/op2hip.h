@@ -94,6 +94,17 @@ int op2hip_plan_verify(int nelems, int nblocks, const int* elems, const int* off
                        const int* const* maps, const int* dims, const int* idxs,
                        const int* spaces, int nspaces, const int* space_size, int* ok);
 
+/* ---- data layout ----------------------------------------------------------------
+ * No counterpart in the reference, which stores every Set in user order.
+ * Orderings that renumber a Set's device storage for locality (user-visible
+ * numbering is unchanged).  perm[pos] = element stored at position pos.
+ * bfs: breadth-first order of a map's targets (adjacent when one iteration
+ * element names both); first_touch: targets in order of first use by the
+ * source elements visited in src_order (NULL: index order). */
+int op2hip_order_bfs(int ntarget, int nelem, const int* map, int dim, int* perm);
+int op2hip_order_first_touch(int ntarget, int nsrc, const int* map, int dim,
+                             const int* src_order, int* perm);
+
 /* ---- sparsity ------------------------------------------------------------------
  * Replaces build_sparsity_pattern_seq (pyop2/sparsity_utils.h:10-13,
  * sparsity_utils.cxx:5-51) as called by build_sparsity

@@ -140,20 +140,43 @@ void grow_regions(int n, int ps, const std::vector<RaceMap>& maps,
     q.push_back(seed);
     region[seed] = r;
     size_t head = 0;
-    while (head < q.size()) {
-      int x = q[head++];
-      for_targets(maps, x, [&](int s, int t) {
-        for (int k = ptr[s][t]; k < ptr[s][t + 1]; ++k) {
-          int y = adj[s][k];
-          if (region[y] >= 0) continue;
-          if (static_cast<int>(q.size()) < ps) {
-            region[y] = r;
-            q.push_back(y);
-          } else {
-            frontier.push_back(y);
+    while (true) {
+      while (head < q.size()) {
+        int x = q[head++];
+        for_targets(maps, x, [&](int s, int t) {
+          for (int k = ptr[s][t]; k < ptr[s][t + 1]; ++k) {
+            int y = adj[s][k];
+            if (region[y] >= 0) continue;
+            if (static_cast<int>(q.size()) < ps) {
+              region[y] = r;
+              q.push_back(y);
+            } else {
+              frontier.push_back(y);
+            }
           }
+        });
+      }
+      // the region's component is exhausted before it is full: keep filling
+      // it from the next unassigned element, so blocks stay ps elements
+      if (static_cast<int>(q.size()) >= ps ||
+          static_cast<int>(order.size() + q.size()) >= n)
+        break;
+      int next = -1;
+      while (!frontier.empty()) {
+        int x = frontier.front();
+        frontier.pop_front();
+        if (region[x] < 0) {
+          next = x;
+          break;
         }
-      });
+      }
+      if (next < 0) {
+        while (next_scan < n && region[next_scan] >= 0) ++next_scan;
+        if (next_scan >= n) break;
+        next = next_scan;
+      }
+      region[next] = r;
+      q.push_back(next);
     }
     std::sort(q.begin(), q.end());
     order.insert(order.end(), q.begin(), q.end());
@@ -377,6 +400,92 @@ int op2hip_plan_verify(int nelems, int nblocks, const int* elems, const int* off
       });
     }
   }
+  return OP2HIP_OK;
+}
+
+// ---- data layout orderings ---------------------------------------------------
+
+// Breadth-first (Cuthill-McKee style) ordering of the targets of a map: two
+// targets are adjacent when one iteration element names both.  Starts from a
+// pseudo-peripheral target (the last one reached by a first sweep), restarts
+// at the lowest unvisited target for every further component.  perm[pos] is
+// the target placed at position pos.  Gives mesh-local data contiguous
+// positions, so gathers through the map reuse cache lines.
+int op2hip_order_bfs(int ntarget, int nelem, const int* map, int dim, int* perm) {
+  OP2HIP_REQUIRE(ntarget >= 0 && nelem >= 0 && dim > 0 && perm, "bad ordering arguments");
+  std::vector<int> ptr(ntarget + 1, 0), adj(static_cast<size_t>(nelem) * dim);
+  for (long long k = 0; k < static_cast<long long>(nelem) * dim; ++k) {
+    int t = map[k];
+    OP2HIP_REQUIRE(t >= 0 && t < ntarget, "map value %d out of range [0,%d)", t, ntarget);
+    ptr[t + 1]++;
+  }
+  for (int t = 0; t < ntarget; ++t) ptr[t + 1] += ptr[t];
+  {
+    std::vector<int> cur(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < nelem; ++e)
+      for (int d = 0; d < dim; ++d) adj[cur[map[static_cast<long long>(e) * dim + d]]++] = e;
+  }
+  std::vector<int> pos(ntarget, -1);
+  std::vector<int> q;
+  q.reserve(ntarget);
+  auto bfs = [&](int start, std::vector<int>& order, std::vector<int>& mark, int stamp) {
+    size_t head = order.size();
+    order.push_back(start);
+    mark[start] = stamp;
+    while (head < order.size()) {
+      int t = order[head++];
+      for (int k = ptr[t]; k < ptr[t + 1]; ++k) {
+        const int* row = map + static_cast<long long>(adj[k]) * dim;
+        for (int d = 0; d < dim; ++d) {
+          int u = row[d];
+          if (mark[u] != stamp) {
+            mark[u] = stamp;
+            order.push_back(u);
+          }
+        }
+      }
+    }
+  };
+  std::vector<int> mark(ntarget, -1);
+  int next = 0;
+  int comp = 0;
+  while (static_cast<int>(q.size()) < ntarget) {
+    while (next < ntarget && pos[next] >= 0) ++next;
+    // first sweep from `next` to find a far end of its component
+    std::vector<int> sweep;
+    bfs(next, sweep, mark, 2 * comp);
+    int start = sweep.back();
+    size_t before = q.size();
+    bfs(start, q, mark, 2 * comp + 1);
+    for (size_t k = before; k < q.size(); ++k) pos[q[k]] = static_cast<int>(k);
+    ++comp;
+  }
+  memcpy(perm, q.data(), sizeof(int) * ntarget);
+  return OP2HIP_OK;
+}
+
+// First-touch ordering of the targets of a map from an ordered source set:
+// walking the source elements in `src_order`, each target takes the next
+// position when first named; targets never named follow in index order.
+int op2hip_order_first_touch(int ntarget, int nsrc, const int* map, int dim,
+                             const int* src_order, int* perm) {
+  OP2HIP_REQUIRE(ntarget >= 0 && nsrc >= 0 && dim > 0 && perm, "bad ordering arguments");
+  std::vector<char> seen(ntarget, 0);
+  int k = 0;
+  for (int i = 0; i < nsrc; ++i) {
+    int e = src_order ? src_order[i] : i;
+    const int* row = map + static_cast<long long>(e) * dim;
+    for (int d = 0; d < dim; ++d) {
+      int t = row[d];
+      OP2HIP_REQUIRE(t >= 0 && t < ntarget, "map value %d out of range [0,%d)", t, ntarget);
+      if (!seen[t]) {
+        seen[t] = 1;
+        perm[k++] = t;
+      }
+    }
+  }
+  for (int t = 0; t < ntarget; ++t)
+    if (!seen[t]) perm[k++] = t;
   return OP2HIP_OK;
 }
 

@@ -31,6 +31,7 @@ from . import configuration as cfg
 from . import _native as nat
 from . import jit
 from . import cg
+from . import layout
 from .base import (READ, WRITE, RW, INC, MIN, MAX, IdentityMap, IterationIndex, i,
                    IterationSpace, Halo, Access)
 from .codegen import LoopSource
@@ -100,25 +101,44 @@ def _d2h(host, dev):
 
 
 class _DeviceData(object):
-    """Host/device coherence of one array (Dat, Global)."""
+    """Host/device coherence of one array (Dat, Global).  A Dat's device
+    copy is stored in its Set's layout order (see :mod:`layout`)."""
 
     def _dev_init(self):
         self._dev = None
         self._state = HOST
 
+    def _layout(self):
+        ds = getattr(self, '_dataset', None)
+        return layout.of(ds) if ds is not None else layout.IDENTITY
+
+    def _upload(self):
+        _h2d(self._dev, self._layout().to_device_order(self._data))
+
     def _to_device(self):
         if self._dev is None:
             self._dev = _dev_bytes(self._data.nbytes)
-            _h2d(self._dev, self._data)
+            self._upload()
             self._state = BOTH
         elif self._state == HOST:
-            _h2d(self._dev, self._data)
+            self._upload()
             self._state = BOTH
         return self._dev
 
     def _sync_to_host(self):
         if getattr(self, '_state', HOST) == DEVICE:
-            _d2h(self._data, self._dev)
+            lay = self._layout()
+            if lay.identity:
+                _d2h(self._data, self._dev)
+            else:
+                tmp = np.empty_like(self._data)
+                _d2h(tmp, self._dev)
+                w = self._data.flags.writeable
+                if not w and self._data.flags['OWNDATA']:
+                    self._data.setflags(write=True)
+                lay.to_user_order(tmp, self._data)
+                if not w and self._data.flags['OWNDATA']:
+                    self._data.setflags(write=False)
             self._state = BOTH
 
     def _host_written(self):
@@ -138,10 +158,19 @@ class Set(base.Set):
 
 class Map(base.Map):
 
+    def __init__(self, *args, **kwargs):
+        base.Map.__init__(self, *args, **kwargs)
+        if self is not IdentityMap:
+            layout.register_map(self)
+
+    def _layout_values(self):
+        """Map values in device numbering (host int32)."""
+        return layout.map_values(self)
+
     def _device_values(self):
         d = getattr(self, '_dev', None)
         if d is None:
-            v = np.ascontiguousarray(self._values, dtype=np.int32)
+            v = self._layout_values()
             d = _dev_bytes(v.nbytes)
             _h2d(d, v)
             self._dev = d
@@ -172,7 +201,11 @@ class Kernel(base.Kernel):
 
 class Sparsity(base.Sparsity):
     """CSR structure built on the device (bit-exact to the reference's
-    sparsity_utils.cxx:5-51 std::set construction)."""
+    sparsity_utils.cxx:5-51 std::set construction).
+
+    The device CSR is in device numbering (rows and columns in the row and
+    column Sets' layouts); ``rowptr``/``colidx``/``nnz`` are the same
+    pattern in user numbering, derived on first access."""
 
     _cache = {}
 
@@ -181,12 +214,12 @@ class Sparsity(base.Sparsity):
         nrows = self._nrows
         lsize = nrows * rmult
         s = stream()
-        rmaps = [m._device_values() for m in self._rmaps]
-        cmaps = [m._device_values() for m in self._cmaps]
         for m in self._rmaps + self._cmaps:
             v = m.values
             if v.size and (v.min() < 0 or v.max() >= m.dataset.total_size):
                 raise base.MapValueError("Map %s has values outside its dataset" % m.name)
+        rmaps = [m._device_values() for m in self._rmaps]
+        cmaps = [m._device_values() for m in self._cmaps]
         rowptr = torch.empty(lsize + 1, dtype=torch.int32, device=device())
         handle = ctypes.c_void_p()
         nnz = ctypes.c_longlong()
@@ -205,12 +238,53 @@ class Sparsity(base.Sparsity):
                                                ctypes.c_void_p(s)), 'sparsity fill')
         self._dev_rowptr = rowptr
         self._dev_colidx = colidx
-        self._rowptr = rowptr.cpu().numpy()
-        self._colidx = colidx[:nnz.value].cpu().numpy()
-        self._d_nnz = nnz_rows[:lsize].cpu().numpy()
+        self._dev_nnz_rows = nnz_rows
         self._d_nz = int(nnz.value)
         self._o_nnz = np.zeros(lsize, dtype=np.int32)
         self._o_nz = 0
+        self._user = None
+
+    def _row_layout(self):
+        return layout.of(self._rmaps[0].dataset)
+
+    def _col_layout(self):
+        return layout.of(self._cmaps[0].dataset)
+
+    def _user_csr(self):
+        """(rowptr, colidx, nnz, slotmap) in user numbering; slotmap[k] is
+        the device CSR slot of user slot k (None when both layouts are the
+        identity)."""
+        if self._user is not None:
+            return self._user
+        rmult, cmult = self._dims
+        lsize = self._nrows * rmult
+        nz = self._d_nz
+        rp = self._dev_rowptr.cpu().numpy()
+        ci = self._dev_colidx[:nz].cpu().numpy()
+        self._user = layout.csr_to_user(rp, ci, rmult, cmult, self._row_layout(),
+                                        self._col_layout())
+        return self._user
+
+    @property
+    def _rowptr(self):
+        return self._user_csr()[0]
+
+    @property
+    def _colidx(self):
+        return self._user_csr()[1]
+
+    @property
+    def _d_nnz(self):
+        return self._user_csr()[2]
+
+    def _device_rows(self, rows):
+        """Device row numbers of user rows (expanded numbering)."""
+        rmult = self._dims[0]
+        r = np.asarray(rows, dtype=np.int64).reshape(-1)
+        rl = self._row_layout()
+        if rl.identity:
+            return r
+        return rmult * rl.iperm[r // rmult] + r % rmult
 
 
 class Mat(base.Mat):
@@ -245,8 +319,8 @@ class Mat(base.Mat):
         nrows = self._sparsity.nrows * self.dims[0]
         if r.min() < 0 or r.max() >= nrows:
             raise IndexError("row out of range in zero_rows")
-        d = torch.from_numpy(r).to(device())
         sp = self._sparsity
+        d = torch.from_numpy(sp._device_rows(r).astype(np.int32)).to(device())
         nat.check(nat.lib.op2hip_csr_zero_rows(int(r.size), ctypes.c_void_p(_ptr(d)),
                                                ctypes.c_void_p(_ptr(sp._dev_rowptr)),
                                                ctypes.c_void_p(_ptr(sp._dev_colidx)),
@@ -262,7 +336,9 @@ class Mat(base.Mat):
     def array(self):
         """Host copy of the CSR values."""
         self._check_err()
-        return self._dev_vals[:self._sparsity.nz].cpu().numpy()
+        vals = self._dev_vals[:self._sparsity.nz].cpu().numpy()
+        slot = self._sparsity._user_csr()[3]
+        return vals if slot is None else vals[slot]
 
     @property
     def values(self):
@@ -450,7 +526,7 @@ def get_plan(iterset, n, args, part_size=None, reorder=None):
     p = _plan_cache.get(key)
     if p is None:
         p = Plan(n, part_size, reorder,
-                 [(r[0].values[:n], r[0].dim, r[1], r[2], r[3]) for r in uniq])
+                 [(r[0]._layout_values()[:n], r[0].dim, r[1], r[2], r[3]) for r in uniq])
         p._keep = [r[0] for r in uniq]
         _plan_cache[key] = p
     return p
@@ -500,7 +576,14 @@ class _LaunchEntry(object):
         self.loop = loop
         args = pl.args
         n = pl._it_space.exec_size if pl.needs_exec_halo else pl._it_space.size
-        self.keep = [pl.kernel, args, pl._it_space.iterset]
+        iterset = pl._it_space.iterset
+        self.keep = [pl.kernel, args, iterset]
+        # direct Dats on another Set of the same size share the iteration
+        # set's layout (element e is one position in both)
+        for a in args:
+            if a._is_direct and a.data.dataset is not iterset:
+                layout.link(a.data.dataset, iterset)
+        layout.of(iterset)
         self.to_device = []
         for a in args:
             if (a._is_dat or a._is_global) and a.data not in self.to_device:
