@@ -139,7 +139,7 @@ def main():
                    'pct_of_hbm_peak': round(100.0 * value / HBM_PEAK_GBS, 2),
                    'graph': bool(args.graph),
                    'partition_size': hip.cfg.get('hip_partition_size') or 'auto'},
-        'roofline': {'bound': 'hbm', 'kernel': 'op2_res_calc_coloured (one res_calc par_loop)',
+        'roofline': {'bound': 'hbm', 'kernel': 'op2_res_calc_owner (one res_calc par_loop)',
                      'achieved': round(res_gbs, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(res_gbs / HBM_PEAK_GBS, 4),
                      'avg_us': round(res_avg_s * 1e6, 2),

@@ -94,6 +94,21 @@ int op2hip_plan_verify(int nelems, int nblocks, const int* elems, const int* off
                        const int* const* maps, const int* dims, const int* idxs,
                        const int* spaces, int nspaces, const int* space_size, int* ok);
 
+/* Owner-computes plan for loops whose conflicting writes are increments into
+ * one target set (also replaces pyop2/plan.pyx:68-456 for those loops).
+ * Targets are owned by blocks in contiguous ranges of k; a block runs every
+ * element naming one of its targets (ascending order) and sums each owned
+ * target's increments in that order.  Arrays (which): 0 blk_t0, 1 blk_t1,
+ * 2 ex_off, 3 exec (bit 31: not primary), 4 tp, 5 contrib (slot*nac + ac).
+ * Scalars (field): 0 nactive, 1 max_exec, 2 nac, 3 exec length, 4 contrib
+ * length. */
+int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* const* maps,
+                            const int* dims, const int* idxs, int k, const int* elem_order,
+                            void** plan_out);
+int op2hip_owner_plan_scalar(void* plan, int field, int64_t* out);
+int op2hip_owner_plan_array(void* plan, int which, int* dst);
+int op2hip_owner_plan_destroy(void* plan);
+
 /* ---- data layout ----------------------------------------------------------------
  * No counterpart in the reference, which stores every Set in user order.
  * Orderings that renumber a Set's device storage for locality (user-visible

@@ -141,7 +141,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
 '''
 
     def __init__(self, kernel, extents, args, consts, mode, block_size):
-        assert mode in ('stream', 'coloured')
+        assert mode in ('stream', 'coloured', 'owner')
         self.kernel = kernel
         self.extents = tuple(extents)
         self.infos = [ArgInfo(k, a, self.extents) for k, a in enumerate(args)]
@@ -167,6 +167,9 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         if self.mode == 'coloured':
             roles += [('plan_elems', None), ('plan_blkmap', None), ('plan_offset', None),
                       ('plan_nelem', None), ('plan_nthrcol', None), ('plan_thrcol', None)]
+        elif self.mode == 'owner':
+            roles += [('own_t0', None), ('own_t1', None), ('own_exoff', None),
+                      ('own_exec', None), ('own_tp', None), ('own_contrib', None)]
         for a in self.infos:
             for _, role in a.params():
                 roles.append((role, a.pos))
@@ -245,7 +248,13 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             if a.kind == 'gbl_read':
                 call_args.append('g%d' % p)
             elif a.kind == 'gbl_red':
-                call_args.append('red%d' % p)
+                if self.mode == 'owner':
+                    decls.append('%s junk%d[%d];' % (t, p, a.cdim))
+                    decls.append('for (int k = 0; k < %d; ++k) junk%d[k] = red%d[k];'
+                                 % (a.cdim, p, p))
+                    call_args.append('(prim ? red%d : junk%d)' % (p, p))
+                else:
+                    call_args.append('red%d' % p)
             elif a.kind == 'direct':
                 call_args.append('d%d + (op2_i64)e * %d' % (p, a.cdim))
             elif a.kind == 'ind':
@@ -359,12 +368,73 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                 lines.append('}')
         return lines
 
+    def owner_columns(self):
+        """(arg info, first race column, number of columns) of the race
+        arguments, in the race-column order of the owner plan."""
+        out, ac = [], 0
+        for a in self.infos:
+            if not a.race:
+                continue
+            n = 1 if a.kind == 'ind' else (a.mdim if a.kind == 'vec' else a.extent)
+            out.append((a, ac, n))
+            ac += n
+        return out
+
+    def _owner_body(self, decls, compute):
+        ind = lambda lines, n: '\n'.join(' ' * n + l for l in lines)
+        bs = self.bs
+        cols = self.owner_columns()
+        nac = sum(n for _, _, n in cols)
+        tgt = cols[0][0]
+        t, cdim = tgt.ctype, tgt.cdim
+        store = []
+        for a, ac0, n in cols:
+            if a.kind == 'ind':
+                store.append('for (int k = 0; k < %d; ++k) op2_lds[(l * %d + %d) * %d + k] = inc%d[k];'
+                             % (cdim, nac, ac0, cdim, a.pos))
+            else:
+                store.append('for (int j = 0; j < %d; ++j) for (int k = 0; k < %d; ++k) '
+                             'op2_lds[(l * %d + %d + j) * %d + k] = inc%d[j][k];'
+                             % (n, cdim, nac, ac0, cdim, a.pos))
+        out = ['  extern __shared__ %s op2_lds[];' % t,
+               ind(self._red_init(), 2),
+               '''  // consecutive owner blocks (neighbouring target ranges) go to one XCD
+  const int nb = (int)count, bid = blockIdx.x;
+  const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
+  const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
+  const int ex0 = p_exoff[b], nex = p_exoff[b + 1] - ex0;
+  for (int l = threadIdx.x; l < nex; l += %d) {
+    const unsigned code = (unsigned)p_exec[ex0 + l];
+    const int e = (int)(code & 0x7fffffffu);
+    const bool prim = (code >> 31) == 0u;
+    (void)prim;''' % bs,
+               ind(decls + compute + store, 4),
+               '  }',
+               '  __syncthreads();',
+               '''  // each owned target: its old value plus its increments in ascending
+  // element order
+  const int t0 = p_t0[b], t1 = p_t1[b];
+  for (int j = threadIdx.x; j < (t1 - t0) * %(cdim)d; j += %(bs)d) {
+    const int t = t0 + j / %(cdim)d, k = j - (j / %(cdim)d) * %(cdim)d;
+    const int cb = p_tp[t], ce = p_tp[t + 1];
+    if (cb == ce) continue;
+    %(t)s acc = d%(pos)d[(op2_i64)t * %(cdim)d + k];
+    for (int c = cb; c < ce; ++c) acc += op2_lds[p_contrib[c] * %(cdim)d + k];
+    d%(pos)d[(op2_i64)t * %(cdim)d + k] = acc;
+  }''' % {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos},
+               ind(self._red_finish('b'), 2)]
+        return out
+
     def _params_decl(self):
         decl = ['op2_i64 start', 'op2_i64 count']
         if self.mode == 'coloured':
             decl += ['const int* __restrict__ p_elems', 'const int* __restrict__ p_blkmap',
                      'const int* __restrict__ p_offset', 'const int* __restrict__ p_nelem',
                      'const int* __restrict__ p_nthrcol', 'const int* __restrict__ p_thrcol']
+        elif self.mode == 'owner':
+            decl += ['const int* __restrict__ p_t0', 'const int* __restrict__ p_t1',
+                     'const int* __restrict__ p_exoff', 'const int* __restrict__ p_exec',
+                     'const int* __restrict__ p_tp', 'const int* __restrict__ p_contrib']
         for a in self.infos:
             decl += [d for d, _ in a.params()]
         if self.has_mats:
@@ -390,6 +460,8 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             out.append(ind(decls + compute, 4))
             out.append('  }')
             out.append(ind(self._red_finish('blockIdx.x'), 2))
+        elif self.mode == 'owner':
+            out += self._owner_body(decls, compute)
         else:
             out.append(ind(self._red_init(), 2))
             out.append('''  // blocks of one colour are spread over the 8 XCDs in contiguous

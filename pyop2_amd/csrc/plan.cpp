@@ -403,6 +403,160 @@ int op2hip_plan_verify(int nelems, int nblocks, const int* elems, const int* off
   return OP2HIP_OK;
 }
 
+// ---- owner-computes plan -------------------------------------------------------
+
+namespace {
+
+struct OwnerPlan {
+  int nelems = 0, ntargets = 0, nac = 0, k = 0;
+  int nactive = 0, max_exec = 0;
+  std::vector<int> blk_t0, blk_t1, ex_off, exec, tp, contrib;
+};
+
+}  // namespace
+
+// Owner-computes plan for a loop whose conflicting writes are increments into
+// one target set.  Targets are owned by blocks in contiguous ranges of k
+// (contiguous positions are mesh-local under the device layout).  A block
+// executes every element that names one of its targets, in ascending element
+// order, and sums the increments of each owned target in that order (the
+// reference's sequential order), so no two blocks write one target and no
+// colouring or atomic is needed.  Elements named by several blocks' targets
+// run once per block; the block owning the element's first target is its
+// primary (the one whose Global contributions count).
+//
+// elem_order (may be NULL: index order) lists the elements in the order
+// their increments are summed: the user numbering when the iteration set is
+// laid out in another order, which reproduces the sequential order exactly.
+//
+// Race columns ("ac"): race map i contributes columns idxs[i] (or all of its
+// dims[i] columns when idxs[i] < 0), numbered in order across maps.
+//   blk_t0/blk_t1[nactive]  owned target range of active block b
+//   ex_off[nactive+1], exec  executed elements; bit 31 set when not primary
+//   tp[ntargets+1], contrib  per target, its (slot * nac + ac) increments in
+//                            ascending element order (slot: block-local)
+int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* const* maps,
+                            const int* dims, const int* idxs, int k, const int* elem_order,
+                            void** plan_out) {
+  OP2HIP_REQUIRE(plan_out && nelems >= 0 && ntargets >= 0 && k > 0 && nmaps > 0,
+                 "bad owner plan arguments");
+  struct Col {
+    const int* v;
+    int dim, col;
+  };
+  std::vector<Col> cols;
+  for (int m = 0; m < nmaps; ++m) {
+    OP2HIP_REQUIRE(maps[m] && dims[m] > 0 && idxs[m] < dims[m], "bad race map %d", m);
+    if (idxs[m] >= 0) {
+      cols.push_back({maps[m], dims[m], idxs[m]});
+    } else {
+      for (int c = 0; c < dims[m]; ++c) cols.push_back({maps[m], dims[m], c});
+    }
+  }
+  const int nac = static_cast<int>(cols.size());
+  auto target = [&](int e, int ac) {
+    const Col& c = cols[ac];
+    return c.v[static_cast<long long>(e) * c.dim + c.col];
+  };
+  // incidences per target, ascending element then column
+  std::vector<int> cnt(ntargets + 1, 0);
+  for (int e = 0; e < nelems; ++e)
+    for (int ac = 0; ac < nac; ++ac) {
+      int t = target(e, ac);
+      OP2HIP_REQUIRE(t >= 0 && t < ntargets, "map value %d out of range [0,%d)", t, ntargets);
+      cnt[t + 1]++;
+    }
+  for (int t = 0; t < ntargets; ++t) cnt[t + 1] += cnt[t];
+  std::vector<long long> inc(static_cast<size_t>(cnt[ntargets]));
+  {
+    std::vector<int> cur(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < nelems; ++i) {
+      int e = elem_order ? elem_order[i] : i;
+      OP2HIP_REQUIRE(e >= 0 && e < nelems, "bad element order entry %d", e);
+      for (int ac = 0; ac < nac; ++ac) {
+        int t = target(e, ac);
+        inc[cur[t]++] = static_cast<long long>(e) * nac + ac;
+      }
+    }
+  }
+  OwnerPlan* p = new OwnerPlan();
+  p->nelems = nelems;
+  p->ntargets = ntargets;
+  p->nac = nac;
+  p->k = k;
+  p->tp.assign(ntargets + 1, 0);
+  p->contrib.reserve(inc.size());
+  p->ex_off.push_back(0);
+  std::vector<int> els;
+  for (int t0 = 0; t0 < ntargets; t0 += k) {
+    int t1 = std::min(ntargets, t0 + k);
+    els.clear();
+    for (int t = t0; t < t1; ++t)
+      for (int q = cnt[t]; q < cnt[t + 1]; ++q) els.push_back(static_cast<int>(inc[q] / nac));
+    std::sort(els.begin(), els.end());
+    els.erase(std::unique(els.begin(), els.end()), els.end());
+    for (int t = t0; t < t1; ++t) {
+      for (int q = cnt[t]; q < cnt[t + 1]; ++q) {
+        int e = static_cast<int>(inc[q] / nac);
+        int ac = static_cast<int>(inc[q] % nac);
+        int slot = static_cast<int>(std::lower_bound(els.begin(), els.end(), e) - els.begin());
+        p->contrib.push_back(slot * nac + ac);
+      }
+      p->tp[t + 1] = static_cast<int>(p->contrib.size());
+    }
+    if (els.empty()) continue;
+    for (int e : els) {
+      bool primary = target(e, 0) / k == t0 / k;
+      p->exec.push_back(primary ? e : static_cast<int>(static_cast<unsigned>(e) | 0x80000000u));
+    }
+    p->blk_t0.push_back(t0);
+    p->blk_t1.push_back(t1);
+    p->ex_off.push_back(static_cast<int>(p->exec.size()));
+    p->max_exec = std::max(p->max_exec, static_cast<int>(els.size()));
+  }
+  p->nactive = static_cast<int>(p->blk_t0.size());
+  *plan_out = p;
+  return OP2HIP_OK;
+}
+
+// 0 nactive, 1 max_exec, 2 nac, 3 exec length, 4 contrib length
+int op2hip_owner_plan_scalar(void* plan, int field, int64_t* out) {
+  OwnerPlan* p = static_cast<OwnerPlan*>(plan);
+  OP2HIP_REQUIRE(p && out, "null plan");
+  switch (field) {
+    case 0: *out = p->nactive; break;
+    case 1: *out = p->max_exec; break;
+    case 2: *out = p->nac; break;
+    case 3: *out = static_cast<int64_t>(p->exec.size()); break;
+    case 4: *out = static_cast<int64_t>(p->contrib.size()); break;
+    default: OP2HIP_REQUIRE(false, "unknown owner plan field %d", field);
+  }
+  return OP2HIP_OK;
+}
+
+// 0 blk_t0, 1 blk_t1, 2 ex_off, 3 exec, 4 tp, 5 contrib
+int op2hip_owner_plan_array(void* plan, int which, int* dst) {
+  OwnerPlan* p = static_cast<OwnerPlan*>(plan);
+  OP2HIP_REQUIRE(p && dst, "null plan");
+  const std::vector<int>* v = nullptr;
+  switch (which) {
+    case 0: v = &p->blk_t0; break;
+    case 1: v = &p->blk_t1; break;
+    case 2: v = &p->ex_off; break;
+    case 3: v = &p->exec; break;
+    case 4: v = &p->tp; break;
+    case 5: v = &p->contrib; break;
+    default: OP2HIP_REQUIRE(false, "unknown owner plan array %d", which);
+  }
+  if (!v->empty()) memcpy(dst, v->data(), v->size() * sizeof(int));
+  return OP2HIP_OK;
+}
+
+int op2hip_owner_plan_destroy(void* plan) {
+  delete static_cast<OwnerPlan*>(plan);
+  return OP2HIP_OK;
+}
+
 // ---- data layout orderings ---------------------------------------------------
 
 // Breadth-first (Cuthill-McKee style) ordering of the targets of a map: two

@@ -532,6 +532,96 @@ def get_plan(iterset, n, args, part_size=None, reorder=None):
     return p
 
 
+class OwnerPlan(object):
+    """Owner-computes plan (csrc/plan.cpp op2hip_owner_plan_build): targets
+    owned by blocks in contiguous ranges of ``k``; each block runs the
+    elements naming its targets and sums each owned target's increments in
+    ascending element order."""
+
+    ARRAYS = ('blk_t0', 'blk_t1', 'ex_off', 'exec', 'tp', 'contrib')
+
+    def __init__(self, n, ntargets, cols, k, elem_order=None):
+        self.n, self.ntargets, self.k = n, ntargets, k
+        keep = [np.ascontiguousarray(v, dtype=np.int32) for v, _, _ in cols]
+        order = None if elem_order is None else np.ascontiguousarray(elem_order, dtype=np.int32)
+        h = ctypes.c_void_p()
+        nat.check(nat.lib.op2hip_owner_plan_build(
+            n, ntargets, len(cols), nat.ptr_array([v.ctypes.data for v in keep]),
+            nat.int_array([d for _, d, _ in cols]), nat.int_array([i for _, _, i in cols]), k,
+            None if order is None else order.ctypes.data_as(ctypes.c_void_p),
+            ctypes.byref(h)), 'owner plan build')
+        try:
+            v = ctypes.c_int64()
+            sc = []
+            for f in range(5):
+                nat.check(nat.lib.op2hip_owner_plan_scalar(h, f, ctypes.byref(v)))
+                sc.append(v.value)
+            self.nactive, self.max_exec, self.nac, nexec, ncontrib = sc
+            sizes = {'blk_t0': self.nactive, 'blk_t1': self.nactive,
+                     'ex_off': self.nactive + 1, 'exec': nexec, 'tp': ntargets + 1,
+                     'contrib': ncontrib}
+            for w, name in enumerate(self.ARRAYS):
+                arr = np.zeros(sizes[name], dtype=np.int32)
+                if arr.size:
+                    nat.check(nat.lib.op2hip_owner_plan_array(
+                        h, w, arr.ctypes.data_as(ctypes.c_void_p)))
+                setattr(self, name, arr)
+        finally:
+            nat.lib.op2hip_owner_plan_destroy(h)
+        self._dev = None
+
+    def device_arrays(self):
+        if self._dev is None:
+            self._dev = {}
+            for name in self.ARRAYS:
+                a = getattr(self, name)
+                d = _dev_bytes(a.nbytes)
+                _h2d(d, a)
+                self._dev[name] = d
+        return self._dev
+
+
+_owner_cache = {}
+OWNER_LDS_LIMIT = 48 * 1024
+
+
+def _owner_columns(args):
+    """(map, idx or -1) of every race argument, in argument order."""
+    cols = []
+    for a in args:
+        if a._is_indirect and a.access is INC:
+            idx = a.idx if isinstance(a.idx, int) else -1
+            cols.append((a.map, idx))
+    return cols
+
+
+def get_owner_plan(iterset, n, args, slot_bytes, k=None):
+    """Cached owner plan; halves the owned range until a block's increments
+    fit ``OWNER_LDS_LIMIT`` bytes of LDS (``slot_bytes`` per element)."""
+    cols = _owner_columns(args)
+    target = cols[0][0].dataset
+    k = k or cfg.get('hip_owner_targets') or 256
+    key = (id(iterset), n, k, tuple((id(m), i) for m, i in cols))
+    p = _owner_cache.get(key)
+    if p is None:
+        kk = k
+        while True:
+            # increments are summed in user element order (the sequential
+            # order): device position of user element i is iperm[i]
+            lay = layout.of(iterset)
+            order = None if lay.identity else lay.iperm[:n]
+            p = OwnerPlan(n, target.total_size,
+                          [(m._layout_values()[:n], m.dim, i) for m, i in cols], kk, order)
+            if p.max_exec * slot_bytes <= OWNER_LDS_LIMIT or kk <= 8:
+                break
+            kk //= 2
+        if p.max_exec * slot_bytes > OWNER_LDS_LIMIT:
+            return None
+        p._keep = [m for m, _ in cols]
+        _owner_cache[key] = p
+    return p
+
+
 # -- loops -------------------------------------------------------------------
 
 class _CompiledLoop(object):
@@ -591,7 +681,19 @@ class _LaunchEntry(object):
         for d in self.to_device:
             d._to_device()
         plan = None
-        if mode == 'coloured':
+        lds = 0
+        if mode == 'owner':
+            cols = loop.source.owner_columns()
+            tgt = cols[0][0]
+            slot_bytes = sum(nc for _, _, nc in cols) * tgt.cdim * \
+                np.dtype(tgt.arg.data.dtype).itemsize
+            plan = get_owner_plan(iterset, n, args, slot_bytes)
+            if plan is None:
+                raise RuntimeError("owner plan does not fit LDS")
+            nparts = max(plan.nactive, 1)
+            lds = plan.max_exec * slot_bytes
+            grid_stream = 0
+        elif mode == 'coloured':
             plan = get_plan(pl._it_space.iterset, n, args,
                             part_size=cfg.get('hip_partition_size') or bs)
             nparts = plan.nblocks
@@ -603,7 +705,7 @@ class _LaunchEntry(object):
         vals = []
         for role, pos in loop.source.param_roles():
             a = args[pos] if pos is not None else None
-            if role in ('start', 'count') or role.startswith('plan_'):
+            if role in ('start', 'count') or role.startswith('plan_') or role.startswith('own_'):
                 vals.append(role)
             elif role == 'gbl':
                 vals.append(_ptr(a.data._dev))
@@ -634,7 +736,16 @@ class _LaunchEntry(object):
         if n > 0:
             if mode == 'stream':
                 argv = [0 if r == 'start' else n if r == 'count' else r for r in vals]
-                launches.append((loop.fn, grid_stream, bs, _argv(argv)))
+                launches.append((loop.fn, grid_stream, bs, _argv(argv), 0))
+            elif mode == 'owner':
+                pdev = plan.device_arrays()
+                self.keep.append(plan)
+                names = {'own_t0': 'blk_t0', 'own_t1': 'blk_t1', 'own_exoff': 'ex_off',
+                         'own_exec': 'exec', 'own_tp': 'tp', 'own_contrib': 'contrib'}
+                argv = [0 if r == 'start' else plan.nactive if r == 'count' else
+                        _ptr(pdev[names[r]]) if isinstance(r, str) else r for r in vals]
+                if plan.nactive:
+                    launches.append((loop.fn, plan.nactive, bs, _argv(argv), lds))
             else:
                 pdev = plan.device_arrays()
                 self.keep.append(plan)
@@ -655,7 +766,7 @@ class _LaunchEntry(object):
                             argv.append(_ptr(pdev[names[r]]))
                         else:
                             argv.append(r)
-                    launches.append((loop.fn, cnt, bs, _argv(argv)))
+                    launches.append((loop.fn, cnt, bs, _argv(argv), 0))
             if loop.reduce_fn is not None:
                 rargv = []
                 for role, pos in loop.source.reduce_roles():
@@ -665,7 +776,7 @@ class _LaunchEntry(object):
                         rargv.append(_ptr(args[pos].data._dev))
                     else:
                         rargv.append(_ptr(partials[pos]))
-                launches.append((loop.reduce_fn, 1, 256, _argv(rargv)))
+                launches.append((loop.reduce_fn, 1, 256, _argv(rargv), 0))
         self.launches = launches
         self.written = [a.data for a in args if (a._is_dat and a.access is not READ) or
                         a._is_global_reduction]
@@ -688,8 +799,8 @@ class _LaunchEntry(object):
                 d._to_device()
         sp = ctypes.c_void_p(s)
         launch = nat.lib.op2hip_launch
-        for fn, grid, bs, (arr, n) in self.launches:
-            st = launch(fn, grid, bs, 0, sp, arr, n)
+        for fn, grid, bs, (arr, n), lds in self.launches:
+            st = launch(fn, grid, bs, lds, sp, arr, n)
             if st != 0:
                 nat.check(st, 'kernel launch')
         for d in self.written:
@@ -726,8 +837,30 @@ def _arg_key(a):
 class ParLoop(base.ParLoop):
 
     def _mode(self):
-        return 'coloured' if any(a._is_mat or (a._is_indirect and a.access is INC)
-                                 for a in self.args) else 'stream'
+        args = self.args
+        if any(a._is_mat for a in args):
+            return 'coloured'
+        race = [a for a in args if a._is_indirect and a.access is INC]
+        if not race:
+            return 'stream'
+        if not cfg.get('hip_owner', True):
+            return 'coloured'
+        # owner-computes: increments into one Dat, every other access a read
+        # (an element may run in two blocks; only its increments to owned
+        # targets and, in its primary block, its Global reductions count)
+        dat = race[0].data
+        if any(a.data is not dat for a in race):
+            return 'coloured'
+        for a in race:
+            if isinstance(a.idx, IterationIndex) and \
+                    self._it_space.extents[a.idx.index] != a.map.dim:
+                return 'coloured'
+        for a in args:
+            if a._is_dat and a not in race and a.access is not READ:
+                return 'coloured'
+            if a._is_dat and a not in race and a.data is dat:
+                return 'coloured'
+        return 'owner'
 
     def _compiled(self, mode, bs, consts):
         key = base.JITModule._cache_key(self.kernel, self._it_space.extents, *self.args)

@@ -76,3 +76,22 @@ def test_repeatable(hip):
         A.remove_consts()
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_owner_mode_bit_exact(pair):
+    """res_calc and bres_calc run owner-computes: each cell's increments are
+    summed onto its old value in ascending edge order, the sequential
+    backend's order, so res (and q after update) match the oracle bit for
+    bit; rms is a tree reduction and matches within tolerance."""
+    A, B = pair
+    for W in (A, B):
+        W.save_soln()
+        W.adt_calc()
+        W.res_calc()
+        W.bres_calc()
+    assert np.array_equal(A.p_adt.data_ro, B.p_adt.data_ro)
+    assert np.array_equal(A.p_res.data_ro, B.p_res.data_ro)
+    for W in (A, B):
+        W.update()
+    assert np.array_equal(A.p_q.data_ro, B.p_q.data_ro)
+    assert abs(A.rms.data[0] - B.rms.data[0]) <= RTOL * abs(B.rms.data[0])

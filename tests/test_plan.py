@@ -95,3 +95,59 @@ def test_airfoil_edges(reorder):
     else:
         # contiguous blocks of a permuted numbering conflict with most others
         assert p.ncolors > 20
+
+
+# -- owner-computes plan --------------------------------------------------------
+
+from pyop2_amd.hip import OwnerPlan  # noqa: E402
+
+
+def check_owner(p, cols, n, ntarget):
+    """cols: list of (values[n, dim], idx).  Every (element, race column)
+    increment appears once, in its target's list, in ascending element
+    order; owned ranges are disjoint; each element is primary once."""
+    colv = []
+    for v, idx in cols:
+        colv += [v[:, idx]] if idx >= 0 else [v[:, c] for c in range(v.shape[1])]
+    nac = len(colv)
+    assert p.nac == nac
+    seen = set()
+    primary = np.zeros(n, dtype=int)
+    owned = np.zeros(ntarget, dtype=int)
+    for b in range(p.nactive):
+        t0, t1 = p.blk_t0[b], p.blk_t1[b]
+        owned[t0:t1] += 1
+        ex = p.exec[p.ex_off[b]:p.ex_off[b + 1]].view(np.uint32)
+        els = (ex & 0x7fffffff).astype(np.int64)
+        assert (np.diff(els) > 0).all()
+        primary[els[(ex >> 31) == 0]] += 1
+        for t in range(t0, t1):
+            prev = -1
+            for c in p.contrib[p.tp[t]:p.tp[t + 1]]:
+                slot, ac = divmod(int(c), nac)
+                e = int(els[slot])
+                assert colv[ac][e] == t
+                assert e >= prev
+                prev = e
+                seen.add((e, ac))
+    assert owned.max() <= 1
+    assert (primary == 1).all()
+    assert len(seen) == n * nac
+
+
+@settings(max_examples=20, deadline=None)
+@given(n=st.integers(1, 500), ntarget=st.integers(1, 120), dim=st.integers(1, 4),
+       k=st.sampled_from([1, 4, 32, 256]), seed=st.integers(0, 1000))
+def test_owner_random(n, ntarget, dim, k, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.integers(0, ntarget, size=(n, dim)).astype(np.int32)
+    cols = [(v, -1)] if seed % 2 else [(v, 0)] + ([(v, dim - 1)] if dim > 1 else [])
+    p = OwnerPlan(n, ntarget, [(c[0], dim, c[1]) for c in cols], k)
+    check_owner(p, cols, n, ntarget)
+
+
+def test_owner_airfoil_edges():
+    m = meshes.QuadMesh(60, 30, seed=5)
+    cols = [(m.pecell, 0), (m.pecell, 1)]
+    p = OwnerPlan(m.nedge, m.ncell, [(v, 2, i) for v, i in cols], 64)
+    check_owner(p, cols, m.nedge, m.ncell)
