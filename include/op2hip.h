@@ -99,9 +99,11 @@ int op2hip_plan_verify(int nelems, int nblocks, const int* elems, const int* off
  * Targets are owned by blocks in contiguous ranges of k; a block runs every
  * element naming one of its targets (ascending order) and sums each owned
  * target's increments in that order.  Arrays (which): 0 blk_t0, 1 blk_t1,
- * 2 ex_off, 3 exec (bit 31: not primary), 4 tp, 5 contrib (slot*nac + ac).
- * Scalars (field): 0 nactive, 1 max_exec, 2 nac, 3 exec length, 4 contrib
- * length. */
+ * 2 ex_off, 3 exec (bit 31: not primary), 4 tp, 5 contrib (slot*nac + ac),
+ * 6 tg_off, 7 tgt (targets receiving increments, per block).  Scalars
+ * (field): 0 nactive, 1 max_exec, 2 nac, 3 exec length, 4 contrib length,
+ * 5 tgt length.  elem_order (NULL: index order) is the order increments
+ * are summed in. */
 int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* const* maps,
                             const int* dims, const int* idxs, int k, const int* elem_order,
                             void** plan_out);

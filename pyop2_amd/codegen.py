@@ -168,8 +168,9 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             roles += [('plan_elems', None), ('plan_blkmap', None), ('plan_offset', None),
                       ('plan_nelem', None), ('plan_nthrcol', None), ('plan_thrcol', None)]
         elif self.mode == 'owner':
-            roles += [('own_t0', None), ('own_t1', None), ('own_exoff', None),
-                      ('own_exec', None), ('own_tp', None), ('own_contrib', None)]
+            roles += [('own_tgoff', None), ('own_tgt', None), ('own_exoff', None),
+                      ('own_exec', None), ('own_tp', None), ('own_contrib', None),
+                      ('own_pitch', None)]
         for a in self.infos:
             for _, role in a.params():
                 roles.append((role, a.pos))
@@ -256,7 +257,10 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                 else:
                     call_args.append('red%d' % p)
             elif a.kind == 'direct':
-                call_args.append('d%d + (op2_i64)e * %d' % (p, a.cdim))
+                if self.mode == 'stream' and self._staged(a):
+                    call_args.append('st%d + threadIdx.x * %d' % (p, self._pitch(a)))
+                else:
+                    call_args.append('d%d + (op2_i64)e * %d' % (p, a.cdim))
             elif a.kind == 'ind':
                 if a.race:
                     decls.append('%s inc%d[%d];' % (t, p, a.cdim))
@@ -368,6 +372,54 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                 lines.append('}')
         return lines
 
+    @staticmethod
+    def _staged(a):
+        """Direct Dats with several components go through LDS in the
+        stream template, so a workgroup's loads and stores of its rows are
+        contiguous instead of one strided row per thread."""
+        return a.kind == 'direct' and a.cdim > 1
+
+    @staticmethod
+    def _pitch(a):
+        # odd row pitch (in elements) spreads a wavefront's rows over LDS banks
+        return a.cdim | 1
+
+    def _stream_body(self, decls, compute):
+        ind = lambda lines, n: '\n'.join(' ' * n + l for l in lines)
+        bs = self.bs
+        staged = [a for a in self.infos if self._staged(a)]
+        out = [ind(self._red_init(), 2)]
+        for a in staged:
+            out.append('  __shared__ %s st%d[%d];' % (a.ctype, a.pos, bs * self._pitch(a)))
+        out.append('  const op2_i64 stride = (op2_i64)gridDim.x * %d;' % bs)
+        out.append('  for (op2_i64 base = (op2_i64)blockIdx.x * %d; base < count; base += stride) {'
+                   % bs)
+        out.append('    const op2_i64 e = start + base + threadIdx.x;')
+        out.append('    const bool active = base + threadIdx.x < count;')
+        out.append('    const int nrow = count - base < %d ? (int)(count - base) : %d;' % (bs, bs))
+        out.append('    (void)nrow;')
+        loads = [a for a in staged if a.access is not WRITE]
+        stores = [a for a in staged if a.access is not READ]
+        for a in loads:
+            out.append('    for (int w = threadIdx.x; w < nrow * %d; w += %d) '
+                       'st%d[(w / %d) * %d + w %% %d] = d%d[(start + base) * %d + w];'
+                       % (a.cdim, bs, a.pos, a.cdim, self._pitch(a), a.cdim, a.pos, a.cdim))
+        if staged:
+            out.append('    __syncthreads();')
+        out.append('    if (active) {')
+        out.append(ind(decls + compute, 6))
+        out.append('    }')
+        if stores:
+            out.append('    __syncthreads();')
+            for a in stores:
+                out.append('    for (int w = threadIdx.x; w < nrow * %d; w += %d) '
+                           'd%d[(start + base) * %d + w] = st%d[(w / %d) * %d + w %% %d];'
+                           % (a.cdim, bs, a.pos, a.cdim, a.pos, a.cdim, self._pitch(a), a.cdim))
+        if staged:
+            out.append('    __syncthreads();')
+        out.append('  }')
+        return out
+
     def owner_columns(self):
         """(arg info, first race column, number of columns) of the race
         arguments, in the race-column order of the owner plan."""
@@ -389,13 +441,15 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         t, cdim = tgt.ctype, tgt.cdim
         store = []
         for a, ac0, n in cols:
+            # LDS holds increments column-major, [ac][k][slot] with a row
+            # pitch of lds_pitch slots: a wavefront's stores are contiguous
             if a.kind == 'ind':
-                store.append('for (int k = 0; k < %d; ++k) op2_lds[(l * %d + %d) * %d + k] = inc%d[k];'
-                             % (cdim, nac, ac0, cdim, a.pos))
+                store.append('for (int k = 0; k < %d; ++k) op2_lds[(%d + k) * pitch + l] = inc%d[k];'
+                             % (cdim, ac0 * cdim, a.pos))
             else:
                 store.append('for (int j = 0; j < %d; ++j) for (int k = 0; k < %d; ++k) '
-                             'op2_lds[(l * %d + %d + j) * %d + k] = inc%d[j][k];'
-                             % (n, cdim, nac, ac0, cdim, a.pos))
+                             'op2_lds[((%d + j) * %d + k) * pitch + l] = inc%d[j][k];'
+                             % (n, cdim, ac0, cdim, a.pos))
         out = ['  extern __shared__ %s op2_lds[];' % t,
                ind(self._red_init(), 2),
                '''  // consecutive owner blocks (neighbouring target ranges) go to one XCD
@@ -403,6 +457,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
   const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
   const int ex0 = p_exoff[b], nex = p_exoff[b + 1] - ex0;
+  const int pitch = (int)lds_pitch;
   for (int l = threadIdx.x; l < nex; l += %d) {
     const unsigned code = (unsigned)p_exec[ex0 + l];
     const int e = (int)(code & 0x7fffffffu);
@@ -411,17 +466,20 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                ind(decls + compute + store, 4),
                '  }',
                '  __syncthreads();',
-               '''  // each owned target: its old value plus its increments in ascending
-  // element order
-  const int t0 = p_t0[b], t1 = p_t1[b];
-  for (int j = threadIdx.x; j < (t1 - t0) * %(cdim)d; j += %(bs)d) {
-    const int t = t0 + j / %(cdim)d, k = j - (j / %(cdim)d) * %(cdim)d;
+               '''  // each owned target that receives increments: its old value plus its
+  // increments in the order of the plan (ascending user element)
+  const int g0 = p_tgoff[b], ng = p_tgoff[b + 1] - g0;
+  for (int j = threadIdx.x; j < ng * %(cdim)d; j += %(bs)d) {
+    const int t = p_tgt[g0 + j / %(cdim)d], k = j - (j / %(cdim)d) * %(cdim)d;
     const int cb = p_tp[t], ce = p_tp[t + 1];
-    if (cb == ce) continue;
     %(t)s acc = d%(pos)d[(op2_i64)t * %(cdim)d + k];
-    for (int c = cb; c < ce; ++c) acc += op2_lds[p_contrib[c] * %(cdim)d + k];
+    for (int c = cb; c < ce; ++c) {
+      const int code = p_contrib[c];
+      const int slot = code / %(nac)d, ac = code - slot * %(nac)d;
+      acc += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
+    }
     d%(pos)d[(op2_i64)t * %(cdim)d + k] = acc;
-  }''' % {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos},
+  }''' % {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos, 'nac': nac},
                ind(self._red_finish('b'), 2)]
         return out
 
@@ -432,9 +490,10 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                      'const int* __restrict__ p_offset', 'const int* __restrict__ p_nelem',
                      'const int* __restrict__ p_nthrcol', 'const int* __restrict__ p_thrcol']
         elif self.mode == 'owner':
-            decl += ['const int* __restrict__ p_t0', 'const int* __restrict__ p_t1',
+            decl += ['const int* __restrict__ p_tgoff', 'const int* __restrict__ p_tgt',
                      'const int* __restrict__ p_exoff', 'const int* __restrict__ p_exec',
-                     'const int* __restrict__ p_tp', 'const int* __restrict__ p_contrib']
+                     'const int* __restrict__ p_tp', 'const int* __restrict__ p_contrib',
+                     'op2_i64 lds_pitch']
         for a in self.infos:
             decl += [d for d, _ in a.params()]
         if self.has_mats:
@@ -453,12 +512,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                    % (bs, self.entry, ',\n    '.join(self._params_decl())))
         decls, compute = self._element_body()
         if self.mode == 'stream':
-            out.append(ind(self._red_init(), 2))
-            out.append('  const op2_i64 stride = (op2_i64)gridDim.x * %d;' % bs)
-            out.append('  for (op2_i64 e = start + (op2_i64)blockIdx.x * %d + threadIdx.x; '
-                       'e < start + count; e += stride) {' % bs)
-            out.append(ind(decls + compute, 4))
-            out.append('  }')
+            out += self._stream_body(decls, compute)
             out.append(ind(self._red_finish('blockIdx.x'), 2))
         elif self.mode == 'owner':
             out += self._owner_body(decls, compute)

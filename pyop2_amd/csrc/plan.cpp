@@ -410,7 +410,7 @@ namespace {
 struct OwnerPlan {
   int nelems = 0, ntargets = 0, nac = 0, k = 0;
   int nactive = 0, max_exec = 0;
-  std::vector<int> blk_t0, blk_t1, ex_off, exec, tp, contrib;
+  std::vector<int> blk_t0, blk_t1, ex_off, exec, tp, contrib, tg_off, tgt;
 };
 
 }  // namespace
@@ -435,6 +435,7 @@ struct OwnerPlan {
 //   ex_off[nactive+1], exec  executed elements; bit 31 set when not primary
 //   tp[ntargets+1], contrib  per target, its (slot * nac + ac) increments in
 //                            ascending element order (slot: block-local)
+//   tg_off[nactive+1], tgt   the targets of each block that receive increments
 int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* const* maps,
                             const int* dims, const int* idxs, int k, const int* elem_order,
                             void** plan_out) {
@@ -487,6 +488,7 @@ int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* cons
   p->tp.assign(ntargets + 1, 0);
   p->contrib.reserve(inc.size());
   p->ex_off.push_back(0);
+  p->tg_off.push_back(0);
   std::vector<int> els;
   for (int t0 = 0; t0 < ntargets; t0 += k) {
     int t1 = std::min(ntargets, t0 + k);
@@ -512,6 +514,9 @@ int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* cons
     p->blk_t0.push_back(t0);
     p->blk_t1.push_back(t1);
     p->ex_off.push_back(static_cast<int>(p->exec.size()));
+    for (int t = t0; t < t1; ++t)
+      if (cnt[t + 1] > cnt[t]) p->tgt.push_back(t);
+    p->tg_off.push_back(static_cast<int>(p->tgt.size()));
     p->max_exec = std::max(p->max_exec, static_cast<int>(els.size()));
   }
   p->nactive = static_cast<int>(p->blk_t0.size());
@@ -519,7 +524,7 @@ int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* cons
   return OP2HIP_OK;
 }
 
-// 0 nactive, 1 max_exec, 2 nac, 3 exec length, 4 contrib length
+// 0 nactive, 1 max_exec, 2 nac, 3 exec length, 4 contrib length, 5 tgt length
 int op2hip_owner_plan_scalar(void* plan, int field, int64_t* out) {
   OwnerPlan* p = static_cast<OwnerPlan*>(plan);
   OP2HIP_REQUIRE(p && out, "null plan");
@@ -529,12 +534,13 @@ int op2hip_owner_plan_scalar(void* plan, int field, int64_t* out) {
     case 2: *out = p->nac; break;
     case 3: *out = static_cast<int64_t>(p->exec.size()); break;
     case 4: *out = static_cast<int64_t>(p->contrib.size()); break;
+    case 5: *out = static_cast<int64_t>(p->tgt.size()); break;
     default: OP2HIP_REQUIRE(false, "unknown owner plan field %d", field);
   }
   return OP2HIP_OK;
 }
 
-// 0 blk_t0, 1 blk_t1, 2 ex_off, 3 exec, 4 tp, 5 contrib
+// 0 blk_t0, 1 blk_t1, 2 ex_off, 3 exec, 4 tp, 5 contrib, 6 tg_off, 7 tgt
 int op2hip_owner_plan_array(void* plan, int which, int* dst) {
   OwnerPlan* p = static_cast<OwnerPlan*>(plan);
   OP2HIP_REQUIRE(p && dst, "null plan");
@@ -546,6 +552,8 @@ int op2hip_owner_plan_array(void* plan, int which, int* dst) {
     case 3: v = &p->exec; break;
     case 4: v = &p->tp; break;
     case 5: v = &p->contrib; break;
+    case 6: v = &p->tg_off; break;
+    case 7: v = &p->tgt; break;
     default: OP2HIP_REQUIRE(false, "unknown owner plan array %d", which);
   }
   if (!v->empty()) memcpy(dst, v->data(), v->size() * sizeof(int));

@@ -22,6 +22,7 @@ hiprtc-compiled loop:
 """
 
 import ctypes
+import hashlib
 import weakref
 
 import numpy as np
@@ -156,6 +157,9 @@ class Set(base.Set):
     pass
 
 
+_map_buffers = {}
+
+
 class Map(base.Map):
 
     def __init__(self, *args, **kwargs):
@@ -171,8 +175,15 @@ class Map(base.Map):
         d = getattr(self, '_dev', None)
         if d is None:
             v = self._layout_values()
-            d = _dev_bytes(v.nbytes)
-            _h2d(d, v)
+            # maps with identical device values share one buffer (airfoil's
+            # pecell and pevcell), so a loop reading both loads one array
+            key = (v.shape, hashlib.blake2b(v.tobytes(), digest_size=16).digest())
+            ref = _map_buffers.get(key)
+            d = ref() if ref is not None else None
+            if d is None:
+                d = _dev_bytes(v.nbytes)
+                _h2d(d, v)
+                _map_buffers[key] = weakref.ref(d)
             self._dev = d
         return d
 
@@ -538,7 +549,7 @@ class OwnerPlan(object):
     elements naming its targets and sums each owned target's increments in
     ascending element order."""
 
-    ARRAYS = ('blk_t0', 'blk_t1', 'ex_off', 'exec', 'tp', 'contrib')
+    ARRAYS = ('blk_t0', 'blk_t1', 'ex_off', 'exec', 'tp', 'contrib', 'tg_off', 'tgt')
 
     def __init__(self, n, ntargets, cols, k, elem_order=None):
         self.n, self.ntargets, self.k = n, ntargets, k
@@ -553,13 +564,13 @@ class OwnerPlan(object):
         try:
             v = ctypes.c_int64()
             sc = []
-            for f in range(5):
+            for f in range(6):
                 nat.check(nat.lib.op2hip_owner_plan_scalar(h, f, ctypes.byref(v)))
                 sc.append(v.value)
-            self.nactive, self.max_exec, self.nac, nexec, ncontrib = sc
+            self.nactive, self.max_exec, self.nac, nexec, ncontrib, ntgt = sc
             sizes = {'blk_t0': self.nactive, 'blk_t1': self.nactive,
                      'ex_off': self.nactive + 1, 'exec': nexec, 'tp': ntargets + 1,
-                     'contrib': ncontrib}
+                     'contrib': ncontrib, 'tg_off': self.nactive + 1, 'tgt': ntgt}
             for w, name in enumerate(self.ARRAYS):
                 arr = np.zeros(sizes[name], dtype=np.int32)
                 if arr.size:
@@ -595,30 +606,44 @@ def _owner_columns(args):
     return cols
 
 
-def get_owner_plan(iterset, n, args, slot_bytes, k=None):
-    """Cached owner plan; halves the owned range until a block's increments
-    fit ``OWNER_LDS_LIMIT`` bytes of LDS (``slot_bytes`` per element)."""
+def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256):
+    """Cached owner plan.  The owned range ``k`` is chosen so a block runs
+    about one element per thread (a first build measures elements run per
+    owned target), and halved until a block's increments fit
+    ``OWNER_LDS_LIMIT`` bytes of LDS (``slot_bytes`` per element)."""
     cols = _owner_columns(args)
     target = cols[0][0].dataset
-    k = k or cfg.get('hip_owner_targets') or 256
-    key = (id(iterset), n, k, tuple((id(m), i) for m, i in cols))
+    k_cfg = k or cfg.get('hip_owner_targets')
+    key = (id(iterset), n, k_cfg, block, tuple((id(m), i) for m, i in cols))
     p = _owner_cache.get(key)
-    if p is None:
-        kk = k
-        while True:
-            # increments are summed in user element order (the sequential
-            # order): device position of user element i is iperm[i]
-            lay = layout.of(iterset)
-            order = None if lay.identity else lay.iperm[:n]
-            p = OwnerPlan(n, target.total_size,
-                          [(m._layout_values()[:n], m.dim, i) for m, i in cols], kk, order)
-            if p.max_exec * slot_bytes <= OWNER_LDS_LIMIT or kk <= 8:
-                break
-            kk //= 2
-        if p.max_exec * slot_bytes > OWNER_LDS_LIMIT:
-            return None
-        p._keep = [m for m, _ in cols]
-        _owner_cache[key] = p
+    if p is not None:
+        return p
+    lay = layout.of(iterset)
+    # increments are summed in user element order (the sequential order):
+    # the device position of user element i is iperm[i]
+    order = None if lay.identity else lay.iperm[:n]
+    vals = [(m._layout_values()[:n], m.dim, i) for m, i in cols]
+
+    def build(kk):
+        return OwnerPlan(n, target.total_size, vals, kk, order)
+
+    if k_cfg:
+        kk = int(k_cfg)
+        p = build(kk)
+    else:
+        kk = 128
+        p = build(kk)
+        if p.nactive:
+            per_target = (p.ex_off[-1] / float(p.nactive)) / kk
+            kk = min(1 << 16, max(8, int(0.9 * block / max(per_target, 1e-9))))
+            p = build(kk)
+    while p.max_exec * slot_bytes > OWNER_LDS_LIMIT and kk > 8:
+        kk //= 2
+        p = build(kk)
+    if p.max_exec * slot_bytes > OWNER_LDS_LIMIT:
+        return None
+    p._keep = [m for m, _ in cols]
+    _owner_cache[key] = p
     return p
 
 
@@ -687,7 +712,7 @@ class _LaunchEntry(object):
             tgt = cols[0][0]
             slot_bytes = sum(nc for _, _, nc in cols) * tgt.cdim * \
                 np.dtype(tgt.arg.data.dtype).itemsize
-            plan = get_owner_plan(iterset, n, args, slot_bytes)
+            plan = get_owner_plan(iterset, n, args, slot_bytes, block=bs)
             if plan is None:
                 raise RuntimeError("owner plan does not fit LDS")
             nparts = max(plan.nactive, 1)
@@ -740,9 +765,10 @@ class _LaunchEntry(object):
             elif mode == 'owner':
                 pdev = plan.device_arrays()
                 self.keep.append(plan)
-                names = {'own_t0': 'blk_t0', 'own_t1': 'blk_t1', 'own_exoff': 'ex_off',
+                names = {'own_tgoff': 'tg_off', 'own_tgt': 'tgt', 'own_exoff': 'ex_off',
                          'own_exec': 'exec', 'own_tp': 'tp', 'own_contrib': 'contrib'}
                 argv = [0 if r == 'start' else plan.nactive if r == 'count' else
+                        plan.max_exec if r == 'own_pitch' else
                         _ptr(pdev[names[r]]) if isinstance(r, str) else r for r in vals]
                 if plan.nactive:
                     launches.append((loop.fn, plan.nactive, bs, _argv(argv), lds))

@@ -132,6 +132,9 @@ def check_owner(p, cols, n, ntarget):
                 seen.add((e, ac))
     assert owned.max() <= 1
     assert (primary == 1).all()
+    touched = np.concatenate([p.tgt[p.tg_off[b]:p.tg_off[b + 1]] for b in range(p.nactive)]) \
+        if p.nactive else np.zeros(0, dtype=np.int32)
+    assert np.array_equal(np.sort(touched), np.nonzero(np.diff(p.tp))[0])
     assert len(seen) == n * nac
 
 
