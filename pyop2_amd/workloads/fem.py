@@ -151,3 +151,63 @@ def mass2d(W, mesh, f, solve_rtol=1e-12, name='m2d'):
     return {'rowptr': np.array(sparsity._rowptr), 'colidx': np.array(sparsity._colidx),
             'mat': np.array(mat.array), 'b': np.array(b.data_ro), 'x': np.array(x.data_ro),
             'its': solver.getIterationNumber()}
+
+
+class Assembly3D(object):
+    """P1 Laplace (coefficient per vertex) and vector-P1 mass (2 dofs per
+    vertex, block CSR) on a tetrahedral mesh in world ``W``: the 3-D
+    assembly workload.  ``laplace()`` / ``vector_mass()`` run the assembly
+    par_loops into ``A`` / ``M``."""
+
+    def __init__(self, W, mesh, kappa, name='a3d'):
+        self.W = W
+        nodes = W.Set(mesh.nvert, 1, name + '_nodes')
+        vnodes = W.Set(mesh.nvert, 2, name + '_vnodes')
+        xnodes = W.Set(mesh.nvert, 3, name + '_xnodes')
+        tets = W.Set(mesh.nelem, 1, name + '_tets')
+        self.nodes, self.vnodes, self.xnodes, self.tets = nodes, vnodes, xnodes, tets
+        self.tet_node = W.Map(tets, nodes, 4, mesh.elem_node, name + '_tet_node')
+        self.tet_vnode = W.Map(tets, vnodes, 4, mesh.elem_node, name + '_tet_vnode')
+        self.tet_xnode = W.Map(tets, xnodes, 4, mesh.elem_node, name + '_tet_xnode')
+        self.coords = W.Dat(xnodes, mesh.coords.copy(), np.float64, name + '_coords')
+        self.kappa = W.Dat(nodes, np.asarray(kappa, dtype=np.float64).copy(), np.float64,
+                           name + '_kappa')
+        self.sp_scalar = W.Sparsity((self.tet_node, self.tet_node), name + '_sp')
+        self.sp_vector = W.Sparsity((self.tet_vnode, self.tet_vnode), name + '_vsp')
+        self.A = W.Mat(self.sp_scalar, np.float64, name + '_A')
+        self.M = W.Mat(self.sp_vector, np.float64, name + '_M')
+        self.k_laplace = W.Kernel(LAPLACE_TET, 'laplace_tet')
+        self.k_vmass = W.Kernel(MASS_VEC_TET, 'mass_vec_tet')
+
+    def laplace(self):
+        W = self.W
+        W.par_loop(self.k_laplace, self.tets(4, 4),
+                   self.A((self.tet_node[W.i[0]], self.tet_node[W.i[1]]), W.INC),
+                   self.coords(self.tet_xnode, W.READ), self.kappa(self.tet_node, W.READ))
+
+    def vector_mass(self):
+        W = self.W
+        W.par_loop(self.k_vmass, self.tets(4, 4),
+                   self.M((self.tet_vnode[W.i[0]], self.tet_vnode[W.i[1]]), W.INC),
+                   self.coords(self.tet_xnode, W.READ))
+
+
+def kuhn_nnz(n):
+    """Closed-form nonzero count of the P1 pattern of the n^3 Kuhn mesh
+    (each vertex couples to itself and its Kuhn neighbours: offsets in
+    {0,1}^3 and {0,-1}^3 and the 6 axis/face-diagonal offsets of the
+    shared main diagonal), counted by brute force over vertex offsets."""
+    offs = set()
+    for t in [(0, 1, 3, 7), (0, 1, 5, 7), (0, 2, 3, 7), (0, 2, 6, 7), (0, 4, 5, 7), (0, 4, 6, 7)]:
+        for a in t:
+            for b in t:
+                pa = (a & 1, a >> 1 & 1, a >> 2 & 1)
+                pb = (b & 1, b >> 1 & 1, b >> 2 & 1)
+                offs.add(tuple(pb[d] - pa[d] for d in range(3)))
+    total = 0
+    for o in offs:
+        cnt = 1
+        for d in range(3):
+            cnt *= (n + 1) - abs(o[d])
+        total += cnt
+    return total
