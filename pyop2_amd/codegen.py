@@ -32,6 +32,8 @@ fixed order combines them with the Global's value: INC adds, MIN/MAX take
 the min/max (reference semantics: pyop2/base.py:216-248).
 """
 
+import numpy as np
+
 from .base import IterationIndex, READ, WRITE, RW, INC, MIN, MAX
 
 
@@ -427,12 +429,27 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         for a in self.infos:
             if not a.race:
                 continue
-            n = 1 if a.kind == 'ind' else (a.mdim if a.kind == 'vec' else a.extent)
+            if a.kind == 'mat':
+                n = a.rarity
+            else:
+                n = 1 if a.kind == 'ind' else (a.mdim if a.kind == 'vec' else a.extent)
             out.append((a, ac, n))
             ac += n
         return out
 
+    def owner_slot_bytes(self):
+        """LDS bytes per executed element in the owner template."""
+        cols = self.owner_columns()
+        a = cols[0][0]
+        if a.kind == 'mat':
+            e0, e1 = self.extents[0], self.extents[1]
+            return e0 * e1 * a.rdim * a.cdim_m * 8 + a.carity * 4
+        nac = sum(n for _, _, n in cols)
+        return nac * a.cdim * np.dtype(a.arg.data.dtype).itemsize
+
     def _owner_body(self, decls, compute):
+        if self.owner_columns()[0][0].kind == 'mat':
+            return self._owner_mat_body(decls, compute)
         ind = lambda lines, n: '\n'.join(' ' * n + l for l in lines)
         bs = self.bs
         cols = self.owner_columns()
@@ -480,6 +497,78 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
     }
     d%(pos)d[(op2_i64)t * %(cdim)d + k] = acc;
   }''' % {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos, 'nac': nac},
+               ind(self._red_finish('b'), 2)]
+        return out
+
+    def _owner_mat_body(self, decls, compute):
+        """Owner-computes Mat assembly: a block owns the rows of a range of
+        row-set nodes.  Phase 1 runs the elements naming its nodes and keeps
+        their local matrices (and column-map rows) in LDS; phase 2 gives each
+        owned node's CSR slots to a wavefront, one slot per lane, and sums the
+        node's element contributions to that slot in ascending user element
+        order (the order addto_vector/addto_scalar reach it sequentially)."""
+        ind = lambda lines, n: '\n'.join(' ' * n + l for l in lines)
+        bs = self.bs
+        a = self.owner_columns()[0][0]
+        p = a.pos
+        e0, e1 = self.extents[0], self.extents[1]
+        rd, cd = a.rdim, a.cdim_m
+        nent = e0 * e1 * rd * cd
+        insert = a.access is WRITE
+        store = []
+        if a.vector:
+            store.append('for (int i0 = 0; i0 < %d; ++i0) for (int i1 = 0; i1 < %d; ++i1) '
+                         'for (int r = 0; r < %d; ++r) for (int c = 0; c < %d; ++c) '
+                         'op2_lds[(((i0 * %d + i1) * %d + r) * %d + c) * pitch + l] = '
+                         'lm%d[i0][i1][r][c];' % (e0, e1, rd, cd, e1, rd, cd, p))
+        else:
+            store.append('for (int i0 = 0; i0 < %d; ++i0) for (int i1 = 0; i1 < %d; ++i1) '
+                         'op2_lds[(i0 * %d + i1) * pitch + l] = lm%d[i0][i1];'
+                         % (e0, e1, e1, p))
+        store.append('for (int j = 0; j < %d; ++j) op2_cmap[j * pitch + l] = '
+                     'mc%d[(op2_i64)e * %d + j];' % (a.carity, p, a.carity))
+        op = '=' if insert else '+='
+        out = ['  extern __shared__ double op2_lds[];',
+               ind(self._red_init(), 2),
+               '''  const int nb = (int)count, bid = blockIdx.x;
+  const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
+  const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
+  const int ex0 = p_exoff[b], nex = p_exoff[b + 1] - ex0;
+  const int pitch = (int)lds_pitch;
+  int* op2_cmap = (int*)(op2_lds + (op2_i64)%(nent)d * pitch);
+  for (int l = threadIdx.x; l < nex; l += %(bs)d) {
+    const unsigned code = (unsigned)p_exec[ex0 + l];
+    const int e = (int)(code & 0x7fffffffu);
+    const bool prim = (code >> 31) == 0u;
+    (void)prim;''' % {'nent': nent, 'bs': bs},
+               ind(decls + compute + store, 4),
+               '  }',
+               '  __syncthreads();',
+               '''  // one wavefront per owned node; lanes over the CSR slots of its rows
+  const int g0 = p_tgoff[b], ng = p_tgoff[b + 1] - g0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int g = wv; g < ng; g += %(nw)d) {
+    const int node = p_tgt[g0 + g];
+    const int cb = p_tp[node], ce = p_tp[node + 1];
+    const int s0 = rp%(p)d[node * %(rd)d], s1 = rp%(p)d[node * %(rd)d + %(rd)d];
+    for (int sl = s0 + lane; sl < s1; sl += 64) {
+      int row = node * %(rd)d;
+      while (row + 1 < node * %(rd)d + %(rd)d && rp%(p)d[row + 1] <= sl) ++row;
+      const int rr2 = row - node * %(rd)d;
+      const int col = ci%(p)d[sl];
+      const int m = col / %(cd)d, cc = col - m * %(cd)d;
+      double acc = d%(p)d[sl];
+      for (int c = cb; c < ce; ++c) {
+        const int code = p_contrib[c];
+        const int slot = code / %(nac)d, i0 = code - slot * %(nac)d;
+        for (int j = 0; j < %(e1)d; ++j) {
+          if (op2_cmap[j * pitch + slot] == m)
+            acc %(op)s op2_lds[(((i0 * %(e1)d + j) * %(rd)d + rr2) * %(cd)d + cc) * pitch + slot];
+        }
+      }
+      d%(p)d[sl] = acc;
+    }
+  }''' % {'nw': bs // 64, 'p': p, 'rd': rd, 'cd': cd, 'nac': a.rarity, 'e1': e1, 'op': op},
                ind(self._red_finish('b'), 2)]
         return out
 

@@ -597,10 +597,13 @@ OWNER_LDS_LIMIT = 48 * 1024
 
 
 def _owner_columns(args):
-    """(map, idx or -1) of every race argument, in argument order."""
+    """(map, idx or -1) of every race argument, in argument order (a Mat's
+    race columns are its row map's)."""
     cols = []
     for a in args:
-        if a._is_indirect and a.access is INC:
+        if a._is_mat:
+            cols.append((a.map[0], -1))
+        elif a._is_indirect and a.access is INC:
             idx = a.idx if isinstance(a.idx, int) else -1
             cols.append((a.map, idx))
     return cols
@@ -615,9 +618,8 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256):
     target = cols[0][0].dataset
     k_cfg = k or cfg.get('hip_owner_targets')
     key = (id(iterset), n, k_cfg, block, tuple((id(m), i) for m, i in cols))
-    p = _owner_cache.get(key)
-    if p is not None:
-        return p
+    if key in _owner_cache:
+        return _owner_cache[key]
     lay = layout.of(iterset)
     # increments are summed in user element order (the sequential order):
     # the device position of user element i is iperm[i]
@@ -627,6 +629,7 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256):
     def build(kk):
         return OwnerPlan(n, target.total_size, vals, kk, order)
 
+    limit = cfg.get('hip_owner_lds') or OWNER_LDS_LIMIT
     if k_cfg:
         kk = int(k_cfg)
         p = build(kk)
@@ -635,12 +638,18 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256):
         p = build(kk)
         if p.nactive:
             per_target = (p.ex_off[-1] / float(p.nactive)) / kk
-            kk = min(1 << 16, max(8, int(0.9 * block / max(per_target, 1e-9))))
+            kk = min(1 << 16, max(1, int(0.9 * block / max(per_target, 1e-9))))
+            kk = max(1, min(kk, int(limit / slot_bytes / max(per_target, 1e-9))))
             p = build(kk)
-    while p.max_exec * slot_bytes > OWNER_LDS_LIMIT and kk > 8:
-        kk //= 2
+    while p.max_exec * slot_bytes > limit and kk > 1:
+        kk = max(1, int(kk * 0.8))
         p = build(kk)
-    if p.max_exec * slot_bytes > OWNER_LDS_LIMIT:
+    if p.max_exec * slot_bytes > limit:
+        _owner_cache[key] = None
+        return None
+    if p.nactive and p.ex_off[-1] / float(p.nactive) < 64 and not k_cfg:
+        # blocks would run too few elements to keep a workgroup busy
+        _owner_cache[key] = None
         return None
     p._keep = [m for m, _ in cols]
     _owner_cache[key] = p
@@ -672,6 +681,10 @@ class JITModule(base.JITModule):
 
 def _block_size():
     return cfg.get('hip_block_size') or 256
+
+
+class _NoOwnerPlan(Exception):
+    """The loop's owner plan would not fit LDS with enough work per block."""
 
 
 class _LaunchEntry(object):
@@ -708,13 +721,10 @@ class _LaunchEntry(object):
         plan = None
         lds = 0
         if mode == 'owner':
-            cols = loop.source.owner_columns()
-            tgt = cols[0][0]
-            slot_bytes = sum(nc for _, _, nc in cols) * tgt.cdim * \
-                np.dtype(tgt.arg.data.dtype).itemsize
+            slot_bytes = loop.source.owner_slot_bytes()
             plan = get_owner_plan(iterset, n, args, slot_bytes, block=bs)
             if plan is None:
-                raise RuntimeError("owner plan does not fit LDS")
+                raise _NoOwnerPlan()
             nparts = max(plan.nactive, 1)
             lds = plan.max_exec * slot_bytes
             grid_stream = 0
@@ -864,22 +874,34 @@ class ParLoop(base.ParLoop):
 
     def _mode(self):
         args = self.args
-        if any(a._is_mat for a in args):
-            return 'coloured'
+        mats = [a for a in args if a._is_mat]
         race = [a for a in args if a._is_indirect and a.access is INC]
-        if not race:
+        if not mats and not race:
             return 'stream'
         if not cfg.get('hip_owner', True):
             return 'coloured'
-        # owner-computes: increments into one Dat, every other access a read
-        # (an element may run in two blocks; only its increments to owned
-        # targets and, in its primary block, its Global reductions count)
+        ext = self._it_space.extents
+        if mats:
+            # owner-computes Mat assembly: one Mat, no Dat increments, the
+            # iteration space spanning both maps' arities
+            if len(mats) > 1 or race or len(ext) != 2:
+                return 'coloured'
+            a = mats[0]
+            if ext[0] != a.map[0].dim or ext[1] != a.map[1].dim:
+                return 'coloured'
+            if a.idx[0].index != 0 or a.idx[1].index != 1:
+                return 'coloured'
+            if any(x._is_dat and x.access is not READ for x in args):
+                return 'coloured'
+            return 'owner'
+        # owner-computes Dat increments: into one Dat, every other access a
+        # read (an element may run in two blocks; only its increments to
+        # owned targets and, in its primary block, its Global reductions count)
         dat = race[0].data
         if any(a.data is not dat for a in race):
             return 'coloured'
         for a in race:
-            if isinstance(a.idx, IterationIndex) and \
-                    self._it_space.extents[a.idx.index] != a.map.dim:
+            if isinstance(a.idx, IterationIndex) and ext[a.idx.index] != a.map.dim:
                 return 'coloured'
         for a in args:
             if a._is_dat and a not in race and a.access is not READ:
@@ -908,7 +930,10 @@ class ParLoop(base.ParLoop):
         if ent is None:
             if len(_call_cache) >= _CALL_CACHE_MAX:
                 _call_cache.pop(next(iter(_call_cache)))
-            ent = _LaunchEntry(self, self._mode(), bs)
+            try:
+                ent = _LaunchEntry(self, self._mode(), bs)
+            except _NoOwnerPlan:
+                ent = _LaunchEntry(self, 'coloured', bs)
             _call_cache[key] = ent
         ent.run()
         self.maybe_set_halo_update_needed()
