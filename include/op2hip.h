@@ -36,6 +36,9 @@ const char* op2hip_last_error(void);
  * from op2.init, pyop2/op2.py:82). */
 int op2hip_init(int device);
 
+/* LDS bytes per workgroup and per compute unit, compute-unit count. */
+int op2hip_device_limits(int device, int* lds_per_block, int* num_cu, int* lds_per_cu);
+
 /* gcnArchName of `device` into `out`. */
 int op2hip_device_arch(int device, char* out, int outlen);
 

@@ -58,6 +58,7 @@ _SIGS = {
     'op2hip_last_error': (c_char_p, []),
     'op2hip_init': (c_int, [c_int]),
     'op2hip_device_arch': (c_int, [c_int, c_char_p, c_int]),
+    'op2hip_device_limits': (c_int, [c_int, P(c_int), P(c_int), P(c_int)]),
     'op2hip_compile': (c_int, [c_char_p, P(c_char_p), c_int, c_char_p, P(c_void), P(c_size),
                                P(c_int)]),
     'op2hip_free_host': (None, [c_void]),

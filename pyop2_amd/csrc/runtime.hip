@@ -100,6 +100,26 @@ int op2hip_init(int device) {
   return OP2HIP_OK;
 }
 
+// Limits used to size launches: LDS bytes per workgroup, compute units,
+// LDS bytes per compute unit.
+int op2hip_device_limits(int device, int* lds_per_block, int* num_cu, int* lds_per_cu) {
+  int v = 0;
+  if (lds_per_block) {
+    OP2HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    *lds_per_block = v;
+  }
+  if (num_cu) {
+    OP2HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device));
+    *num_cu = v;
+  }
+  if (lds_per_cu) {
+    OP2HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                       device));
+    *lds_per_cu = v;
+  }
+  return OP2HIP_OK;
+}
+
 int op2hip_device_arch(int device, char* out, int outlen) {
   hipDeviceProp_t prop;
   OP2HIP_CHECK(hipGetDeviceProperties(&prop, device));

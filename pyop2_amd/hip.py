@@ -56,6 +56,11 @@ def _setup():
     if torch is None or not torch.cuda.is_available():
         raise RuntimeError("the hip backend needs a ROCm GPU (torch.cuda is unavailable)")
     nat.check(nat.lib.op2hip_init(0), 'op2hip_init')
+    lb, ncu, lcu = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    nat.check(nat.lib.op2hip_device_limits(0, ctypes.byref(lb), ctypes.byref(ncu),
+                                           ctypes.byref(lcu)), 'device limits')
+    _state['lds_per_block'], _state['num_cu'], _state['lds_per_cu'] = \
+        lb.value, ncu.value, lcu.value
     _state['device'] = torch.device('cuda', 0)
     torch.cuda.set_device(0)
     _state['initialised'] = True
@@ -593,7 +598,7 @@ class OwnerPlan(object):
 
 
 _owner_cache = {}
-OWNER_LDS_LIMIT = 48 * 1024
+OWNER_LDS_LIMIT = 64 * 1024
 
 
 def _owner_columns(args):
@@ -629,7 +634,7 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256):
     def build(kk):
         return OwnerPlan(n, target.total_size, vals, kk, order)
 
-    limit = cfg.get('hip_owner_lds') or OWNER_LDS_LIMIT
+    limit = cfg.get('hip_owner_lds') or min(OWNER_LDS_LIMIT, _state.get('lds_per_block') or 65536)
     if k_cfg:
         kk = int(k_cfg)
         p = build(kk)
@@ -647,7 +652,7 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256):
     if p.max_exec * slot_bytes > limit:
         _owner_cache[key] = None
         return None
-    if p.nactive and p.ex_off[-1] / float(p.nactive) < 64 and not k_cfg:
+    if p.nactive and p.ex_off[-1] / float(p.nactive) < 32 and not k_cfg:
         # blocks would run too few elements to keep a workgroup busy
         _owner_cache[key] = None
         return None

@@ -79,6 +79,7 @@ void laplace_tet(double A[1][1], double *x[3], double **kappa, int j, int k)
   double c21 = a[0][2] * a[1][0] - a[0][0] * a[1][2];
   double c22 = a[0][0] * a[1][1] - a[0][1] * a[1][0];
   double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+  double idet = 1.0 / det;
   /* rows of J^-T as cofactors / det; reference gradients of the 4 basis
      functions: (-1,-1,-1), e0, e1, e2 */
   const double g[4][3] = {{-1.0, -1.0, -1.0}, {1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
@@ -87,8 +88,8 @@ void laplace_tet(double A[1][1], double *x[3], double **kappa, int j, int k)
     double cr0 = r == 0 ? c00 : r == 1 ? c01 : c02;
     double cr1 = r == 0 ? c10 : r == 1 ? c11 : c12;
     double cr2 = r == 0 ? c20 : r == 1 ? c21 : c22;
-    gj[r] = (cr0 * g[j][0] + cr1 * g[j][1] + cr2 * g[j][2]) / det;
-    gk[r] = (cr0 * g[k][0] + cr1 * g[k][1] + cr2 * g[k][2]) / det;
+    gj[r] = (cr0 * g[j][0] + cr1 * g[j][1] + cr2 * g[j][2]) * idet;
+    gk[r] = (cr0 * g[k][0] + cr1 * g[k][1] + cr2 * g[k][2]) * idet;
   }
   double kap = 0.25 * (kappa[0][0] + kappa[1][0] + kappa[2][0] + kappa[3][0]);
   A[0][0] += kap * (gj[0] * gk[0] + gj[1] * gk[1] + gj[2] * gk[2]) * fabs(det) / 6.0;
