@@ -125,6 +125,22 @@ void mass_tet(double A[1][1], double *x[3], int j, int k)
 """
 
 
+RHS_TET = """
+void rhs_tet(double A[1], double *x[3], double **f, int j)
+{
+  double a[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) a[r][c] = x[c + 1][r] - x[0][r];
+  double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1])
+             - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0])
+             + a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  /* row j of the exact P1 mass matrix |T|/20 (1 + d_jk) applied to f */
+  double s = f[0][0] + f[1][0] + f[2][0] + f[3][0] + f[j][0];
+  A[0] += fabs(det) / 6.0 / 20.0 * s;
+}
+"""
+
+
 def mass2d(W, mesh, f, solve_rtol=1e-12, name='m2d'):
     """P1 mass matrix and right-hand side M f on a triangle mesh, then the
     CG solve of M x = b; returns host copies (the mass2d demo pattern,
