@@ -41,6 +41,15 @@ def parse():
                    help='approximate CPU time budget of the oracle baseline (0: skip)')
     p.add_argument('--partition-size', type=int, default=0)
     p.add_argument('--block-size', type=int, default=0)
+    p.add_argument('--assembly', type=int, default=128,
+                   help='edge count n of the n^3 Kuhn-tet assembly measurement (0: skip)')
+    p.add_argument('--batch', type=int, default=64,
+                   help='meshes in the batched assembly+CG measurement (0: skip)')
+    p.add_argument('--batch-emax', type=float, default=2e5)
+    p.add_argument('--serve', type=int, default=48,
+                   help='requests per arrival rate in the serving measurement (0: skip)')
+    p.add_argument('--serve-pool', type=int, default=4,
+                   help='2-D and 3-D meshes each in the serving pool')
     return p.parse_args()
 
 
@@ -147,7 +156,65 @@ def main():
     if args.cpu_seconds > 0:
         out['cpu_baseline'] = cpu_baseline(mesh, args.cpu_seconds, step_bytes)
     A.remove_consts()
+    traffic = _profiled_traffic()
+    if traffic is not None:
+        out['roofline']['traffic'] = traffic
+    extra = {}
+    if args.assembly:
+        extra['assembly'] = assembly(op2, args.assembly)
+    if args.batch:
+        from pyop2_amd.workloads import suites
+        extra['batched'] = suites.batch_suite(op2, nmesh=args.batch, emax=args.batch_emax)
+    if args.serve:
+        from pyop2_amd.workloads import suites
+        pool = suites.make_pool(op2, n2d=args.serve_pool, n3d=args.serve_pool,
+                                tri_range=(8000, 100000), tet_range=(50000, 200000))
+        extra['serving'] = suites.serve_suite(op2, pool, nreq=args.serve)
+    if extra:
+        out['extra'] = extra
     print(json.dumps(out))
+
+
+def _profiled_traffic():
+    """HBM bytes per res_calc loop measured by rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (calibrated against a copy kernel), when recorded."""
+    path = os.path.join(ROOT, 'profiles', 'res_calc_airfoil_1200x600_traffic.json')
+    try:
+        with open(path) as f:
+            return json.load(f).get('traffic_bytes_per_loop')
+    except (IOError, ValueError):
+        return None
+
+
+def assembly(op2, n):
+    """CSR nonzeros assembled per second on the n^3 Kuhn-tet mesh: P1
+    Laplace with a lognormal coefficient and vector-P1 mass."""
+    import numpy as np
+    import torch
+    from pyop2_amd.workloads import fem, meshes
+    t = time.perf_counter()
+    mesh = meshes.TetMesh(n, seed=0)
+    kappa = np.random.default_rng(1).lognormal(0.0, 1.0, mesh.nvert)
+    a = fem.Assembly3D(op2, mesh, kappa, name='bench3d')
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t
+    res = {'mesh': '%d^3 Kuhn tets' % n, 'tets': mesh.nelem, 'vertices': mesh.nvert,
+           'sparsity_build_s': round(setup, 2),
+           'laplace_nnz': a.sp_scalar.nz, 'laplace_nnz_closed_form': fem.kuhn_nnz(n),
+           'vector_mass_nnz': a.sp_vector.nz}
+    for name, fn, nnz in (('laplace', a.laplace, a.sp_scalar.nz),
+                          ('vector_mass', a.vector_mass, a.sp_vector.nz)):
+        fn()
+        torch.cuda.synchronize()
+        reps = 5
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        res[name + '_ms'] = round(dt * 1e3, 3)
+        res[name + '_nnz_per_s'] = float('%.4g' % (nnz / dt))
+    return res
 
 
 def cpu_baseline(mesh, budget_s, step_bytes):
