@@ -50,11 +50,24 @@ def parse():
                    help='requests per arrival rate in the serving measurement (0: skip)')
     p.add_argument('--serve-pool', type=int, default=4,
                    help='2-D and 3-D meshes each in the serving pool')
+    p.add_argument('--copy-calibration', action='store_true',
+                   help='only run a 256 MiB device copy three times (the reference reading '
+                        'for rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit')
     return p.parse_args()
 
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        import torch
+        n = 256 * 1024 * 1024 // 8
+        a = torch.ones(n, dtype=torch.float64, device='cuda')
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes': n * 8, 'copies': 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU; --gpus %d is not supported (every run uses one "
               "MI355X)" % args.gpus, file=sys.stderr)
