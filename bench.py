@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     p.add_argument('--gpus', type=int, default=1)
-    p.add_argument('--steps', type=int, default=50)
-    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--steps', type=int, default=1000)
+    p.add_argument('--warmup', type=int, default=20)
     p.add_argument('--nx', type=int, default=1200)
     p.add_argument('--ny', type=int, default=600)
     p.add_argument('--graph', type=int, default=1, help='replay each step as a hipGraph')
@@ -41,6 +41,14 @@ def parse():
                    help='approximate CPU time budget of the oracle baseline (0: skip)')
     p.add_argument('--partition-size', type=int, default=0)
     p.add_argument('--block-size', type=int, default=0)
+    p.add_argument('--owner-block-size', type=int, default=0)
+    p.add_argument('--layout-order', default='', help="'bisect' or 'bfs' (default: the backend's)")
+    p.add_argument('--stage-reads', type=int, default=-1,
+                   help='stage read-only multi-component direct Dats through LDS (-1: default)')
+    p.add_argument('--stage-writes', type=int, default=-1,
+                   help='stage written multi-component direct Dats through LDS (-1: default)')
+    p.add_argument('--owner-targets', type=int, default=0,
+                   help='targets owned per block of owner-computes loops (0: sized automatically)')
     p.add_argument('--assembly', type=int, default=128,
                    help='edge count n of the n^3 Kuhn-tet assembly measurement (0: skip)')
     p.add_argument('--batch', type=int, default=64,
@@ -83,6 +91,16 @@ def main():
         opts['hip_partition_size'] = args.partition_size
     if args.block_size:
         opts['hip_block_size'] = args.block_size
+    if args.stage_reads >= 0:
+        opts['hip_stage_reads'] = bool(args.stage_reads)
+    if args.stage_writes >= 0:
+        opts['hip_stage_writes'] = bool(args.stage_writes)
+    if args.layout_order:
+        opts['hip_layout_order'] = args.layout_order
+    if args.owner_block_size:
+        opts['hip_owner_block_size'] = args.owner_block_size
+    if args.owner_targets:
+        opts['hip_owner_targets'] = args.owner_targets
     op2.init(**opts)
     mesh = meshes.QuadMesh(args.nx, args.ny, seed=0)
     A = airfoil.Airfoil(op2, mesh)

@@ -113,15 +113,28 @@ int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* cons
 int op2hip_owner_plan_scalar(void* plan, int field, int64_t* out);
 int op2hip_owner_plan_array(void* plan, int which, int* dst);
 int op2hip_owner_plan_destroy(void* plan);
+/* Scatter program of an owner-computes Mat assembly (no counterpart in the
+ * reference, whose addto_scalar/addto_vector, pyop2/mat_utils.cxx:4-28,
+ * search each entry's row at add time): per node pair of the Mat's pattern,
+ * the (block-local contribution, column position) codes c*e1+j that add
+ * into it, in sequential order.  poff[npairs+1] and prog[ncontrib*e1]
+ * (uint16 when wide == 0, else int32) are caller-allocated host arrays. */
+int op2hip_mat_program(int nactive, const int* blk_t0, const int* blk_t1, const int* ex_off,
+                       const int* exec, const int* tp, const int* contrib, int nac,
+                       const int* cmap, int carity, int e1, int nnodes, const int* rowptr,
+                       const int* colidx, int rd, int cd, int wide, int* poff, void* prog);
 
 /* ---- data layout ----------------------------------------------------------------
  * No counterpart in the reference, which stores every Set in user order.
  * Orderings that renumber a Set's device storage for locality (user-visible
  * numbering is unchanged).  perm[pos] = element stored at position pos.
  * bfs: breadth-first order of a map's targets (adjacent when one iteration
- * element names both); first_touch: targets in order of first use by the
+ * element names both); bisect: recursive bisection of that adjacency by
+ * breadth-first cuts down to ranges of `leaf` targets (every aligned range
+ * a compact patch); first_touch: targets in order of first use by the
  * source elements visited in src_order (NULL: index order). */
 int op2hip_order_bfs(int ntarget, int nelem, const int* map, int dim, int* perm);
+int op2hip_order_bisect(int ntarget, int nelem, const int* map, int dim, int leaf, int* perm);
 int op2hip_order_first_touch(int ntarget, int nsrc, const int* map, int dim,
                              const int* src_order, int* perm);
 

@@ -88,7 +88,11 @@ _SIGS = {
     'op2hip_owner_plan_scalar': (c_int, [c_void, c_int, P(ctypes.c_int64)]),
     'op2hip_owner_plan_array': (c_int, [c_void, c_int, c_void]),
     'op2hip_owner_plan_destroy': (c_int, [c_void]),
+    'op2hip_mat_program': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void, c_int,
+                                   c_void, c_int, c_int, c_int, c_void, c_void, c_int, c_int,
+                                   c_int, c_void, c_void]),
     'op2hip_order_bfs': (c_int, [c_int, c_int, c_void, c_int, c_void]),
+    'op2hip_order_bisect': (c_int, [c_int, c_int, c_void, c_int, c_int, c_void]),
     'op2hip_order_first_touch': (c_int, [c_int, c_int, c_void, c_int, c_void, c_void]),
     'op2hip_sparsity_build': (c_int, [c_int, c_int, c_int, c_int, P(c_void), P(c_int), P(c_void),
                                       P(c_int), P(c_ll), c_void, P(c_void), P(c_ll), c_void]),

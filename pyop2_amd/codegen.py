@@ -37,6 +37,10 @@ import numpy as np
 from .base import IterationIndex, READ, WRITE, RW, INC, MIN, MAX
 
 
+# contribution codes an owner write-back loads together per target
+OWNER_CMAX = 4
+
+
 class ArgInfo(object):
     """How one ``par_loop`` argument is lowered into the loop template."""
 
@@ -142,8 +146,14 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
 }
 '''
 
-    def __init__(self, kernel, extents, args, consts, mode, block_size):
+    def __init__(self, kernel, extents, args, consts, mode, block_size, prefetch=1,
+                 stage_reads=True, stage_writes=True):
         assert mode in ('stream', 'coloured', 'owner')
+        # owner write-back: 1 issues the loads locating a thread's first
+        # target before the elements run (they overlap), 0 after them
+        self.prefetch = prefetch
+        self.stage_reads = stage_reads
+        self.stage_writes = stage_writes
         self.kernel = kernel
         self.extents = tuple(extents)
         self.infos = [ArgInfo(k, a, self.extents) for k, a in enumerate(args)]
@@ -170,9 +180,11 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             roles += [('plan_elems', None), ('plan_blkmap', None), ('plan_offset', None),
                       ('plan_nelem', None), ('plan_nthrcol', None), ('plan_thrcol', None)]
         elif self.mode == 'owner':
-            roles += [('own_tgoff', None), ('own_tgt', None), ('own_exoff', None),
-                      ('own_exec', None), ('own_tp', None), ('own_contrib', None),
-                      ('own_pitch', None)]
+            roles += [('own_t0', None), ('own_t1', None), ('own_tgoff', None),
+                      ('own_tgt', None), ('own_tgcb', None), ('own_exoff', None), ('own_exec', None),
+                      ('own_tp', None), ('own_contrib', None), ('own_pitch', None), ('own_k', None),
+                      ('own_cpitch', None), ('own_poff', None), ('own_prog', None),
+                      ('own_wide', None)]
         for a in self.infos:
             for _, role in a.params():
                 roles.append((role, a.pos))
@@ -244,7 +256,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         Returns (decls, compute): ``decls`` declares and zeroes the private
         arrays and touches no memory, so it can run for idle threads of a
         workgroup; ``compute`` reads maps and data and calls the kernel."""
-        decls, compute, call_args, zeros = [], [], [], []
+        decls, compute, call_args, zeros, post = [], [], [], [], []
         ext = self.extents
         for a in self.infos:
             p, t = a.pos, a.ctype
@@ -261,6 +273,14 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             elif a.kind == 'direct':
                 if self.mode == 'stream' and self._staged(a):
                     call_args.append('st%d + threadIdx.x * %d' % (p, self._pitch(a)))
+                elif self._row_vector(a):
+                    decls.append('op2_row%d pr%d;' % (p, p))
+                    if a.access is not WRITE:
+                        compute.append('pr%d = *(const op2_row%d*)(d%d + (op2_i64)e * %d);'
+                                       % (p, p, p, a.cdim))
+                    call_args.append('pr%d.v' % p)
+                    post.append('*(op2_row%d*)(d%d + (op2_i64)e * %d) = pr%d;'
+                                % (p, p, a.cdim, p))
                 else:
                     call_args.append('d%d + (op2_i64)e * %d' % (p, a.cdim))
             elif a.kind == 'ind':
@@ -323,6 +343,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         compute.append('%s(%s);' % (self.kernel.name, ', '.join(call_args)))
         compute += ['}'] * len(opens)
         compute += closing
+        compute += post
         return decls, compute
 
     def _apply_increments(self):
@@ -374,12 +395,36 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                 lines.append('}')
         return lines
 
-    @staticmethod
-    def _staged(a):
+    def _staged(self, a):
         """Direct Dats with several components go through LDS in the
         stream template, so a workgroup's loads and stores of its rows are
-        contiguous instead of one strided row per thread."""
-        return a.kind == 'direct' and a.cdim > 1
+        contiguous instead of one strided row per thread (strided row
+        stores cost partial-line writes).  Read-only ones are staged only
+        when ``stage_reads`` is set."""
+        if a.kind != 'direct' or a.cdim == 1:
+            return False
+        if a.access is READ:
+            return self.stage_reads
+        return self.stage_writes
+
+    def _row_vector(self, a):
+        """Direct Dats with several components that the kernel writes,
+        when not staged through LDS: the row lives in registers and moves
+        with the widest aligned vector loads and stores (whole rows)."""
+        return (self.mode == 'stream' and a.kind == 'direct' and a.cdim > 1 and
+                a.access is not READ and not self._staged(a))
+
+    def _row_types(self):
+        out = []
+        for a in self.infos:
+            if self._row_vector(a):
+                nbytes = a.cdim * np.dtype(a.arg.data.dtype).itemsize
+                w = 16
+                while nbytes % w:
+                    w //= 2
+                out.append('struct __attribute__((aligned(%d))) op2_row%d { %s v[%d]; };'
+                           % (w, a.pos, a.ctype, a.cdim))
+        return '\n'.join(out)
 
     @staticmethod
     def _pitch(a):
@@ -393,9 +438,13 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         out = [ind(self._red_init(), 2)]
         for a in staged:
             out.append('  __shared__ %s st%d[%d];' % (a.ctype, a.pos, bs * self._pitch(a)))
-        out.append('  const op2_i64 stride = (op2_i64)gridDim.x * %d;' % bs)
-        out.append('  for (op2_i64 base = (op2_i64)blockIdx.x * %d; base < count; base += stride) {'
-                   % bs)
+        # workgroups are dealt to the 8 XCDs round-robin: give each XCD a
+        # contiguous run of chunks, so neighbouring chunks share its L2
+        out.append('  const int nbk = (int)gridDim.x, bid = (int)blockIdx.x;')
+        out.append('  const int xcd = bid & 7, qx = nbk >> 3, rx = nbk & 7;')
+        out.append('  const op2_i64 lb = xcd * qx + (xcd < rx ? xcd : rx) + (bid >> 3);')
+        out.append('  const op2_i64 stride = (op2_i64)nbk * %d;' % bs)
+        out.append('  for (op2_i64 base = lb * %d; base < count; base += stride) {' % bs)
         out.append('    const op2_i64 e = start + base + threadIdx.x;')
         out.append('    const bool active = base + threadIdx.x < count;')
         out.append('    const int nrow = count - base < %d ? (int)(count - base) : %d;' % (bs, bs))
@@ -456,6 +505,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         nac = sum(n for _, _, n in cols)
         tgt = cols[0][0]
         t, cdim = tgt.ctype, tgt.cdim
+        pf = self.prefetch
         store = []
         for a, ac0, n in cols:
             # LDS holds increments column-major, [ac][k][slot] with a row
@@ -467,6 +517,9 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                 store.append('for (int j = 0; j < %d; ++j) for (int k = 0; k < %d; ++k) '
                              'op2_lds[((%d + j) * %d + k) * pitch + l] = inc%d[j][k];'
                              % (n, cdim, ac0, cdim, a.pos))
+        sub = {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos, 'nac': nac, 'cmax': OWNER_CMAX}
+        codes = '''#pragma unroll
+    for (int u = 0; u < %(cmax)d; ++u) w_code[u] = u < w_nc ? p_contrib[w_cb + u] : 0;''' % sub
         out = ['  extern __shared__ %s op2_lds[];' % t,
                ind(self._red_init(), 2),
                '''  // consecutive owner blocks (neighbouring target ranges) go to one XCD
@@ -474,103 +527,172 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
   const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
   const int ex0 = p_exoff[b], nex = p_exoff[b + 1] - ex0;
-  const int pitch = (int)lds_pitch;
-  for (int l = threadIdx.x; l < nex; l += %d) {
+  const int g0 = p_tgoff[b], ng = p_tgoff[b + 1] - g0;
+  // a block whose every owned target receives increments needs no list
+  const int t0 = p_t0[b];
+  const bool dense = ng == p_t1[b] - t0;
+  const int pitch = (int)lds_pitch;''' % sub]
+        if pf >= 1:
+            out.append('''  // write-back work of this thread's first owned target (its list entry
+  // and contribution range), loaded while the elements run; the index is
+  // clamped to the padding entry the device lists carry
+  const int jf = (int)threadIdx.x < ng ? (int)threadIdx.x : ng;
+  const int f_t = p_tgt[g0 + jf], f_cb = p_tgcb[g0 + jf], f_ce = p_tgcb[g0 + jf + 1];''')
+        out += ['''  for (int l = threadIdx.x; l < nex; l += %d) {
     const unsigned code = (unsigned)p_exec[ex0 + l];
     const int e = (int)(code & 0x7fffffffu);
     const bool prim = (code >> 31) == 0u;
     (void)prim;''' % bs,
-               ind(decls + compute + store, 4),
-               '  }',
-               '  __syncthreads();',
-               '''  // each owned target that receives increments: its old value plus its
-  // increments in the order of the plan (ascending user element)
-  const int g0 = p_tgoff[b], ng = p_tgoff[b + 1] - g0;
-  for (int j = threadIdx.x; j < ng * %(cdim)d; j += %(bs)d) {
-    const int t = p_tgt[g0 + j / %(cdim)d], k = j - (j / %(cdim)d) * %(cdim)d;
-    const int cb = p_tp[t], ce = p_tp[t + 1];
-    %(t)s acc = d%(pos)d[(op2_i64)t * %(cdim)d + k];
-    for (int c = cb; c < ce; ++c) {
-      const int code = p_contrib[c];
-      const int slot = code / %(nac)d, ac = code - slot * %(nac)d;
-      acc += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
+                ind(decls + compute + store, 4),
+                '  }',
+                '  __syncthreads();',
+                '''  // each owned target that receives increments, one per thread: its old
+  // value plus its increments in the order of the plan (ascending user
+  // element); contribution codes are loaded %(cmax)d at a time
+  for (int j = threadIdx.x; j < ng; j += %(bs)d) {
+    int w_t, w_cb, w_nc;''' % sub]
+        if pf >= 1:
+            out.append('''    if (j == (int)threadIdx.x) {
+      w_t = dense ? t0 + j : f_t;
+      w_cb = f_cb;
+      w_nc = f_ce - f_cb;
+    } else {
+      w_t = dense ? t0 + j : p_tgt[g0 + j];
+      w_cb = p_tgcb[g0 + j];
+      w_nc = p_tgcb[g0 + j + 1] - w_cb;
+    }''')
+        else:
+            out.append('''    w_t = dense ? t0 + j : p_tgt[g0 + j];
+    w_cb = p_tgcb[g0 + j];
+    w_nc = p_tgcb[g0 + j + 1] - w_cb;''')
+        out.append('    int w_code[%(cmax)d];' % sub)
+        out.append(codes)
+        out.append('''    %(t)s acc[%(cdim)d];
+#pragma unroll
+    for (int k = 0; k < %(cdim)d; ++k) acc[k] = d%(pos)d[(op2_i64)w_t * %(cdim)d + k];
+#pragma unroll
+    for (int u = 0; u < %(cmax)d; ++u) {
+      if (u < w_nc) {
+        const int slot = w_code[u] / %(nac)d, ac = w_code[u] - slot * %(nac)d;
+#pragma unroll
+        for (int k = 0; k < %(cdim)d; ++k) acc[k] += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
+      }
     }
-    d%(pos)d[(op2_i64)t * %(cdim)d + k] = acc;
-  }''' % {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos, 'nac': nac},
-               ind(self._red_finish('b'), 2)]
+    for (int c = %(cmax)d; c < w_nc; ++c) {
+      const int cc = p_contrib[w_cb + c];
+      const int slot = cc / %(nac)d, ac = cc - slot * %(nac)d;
+#pragma unroll
+      for (int k = 0; k < %(cdim)d; ++k) acc[k] += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
+    }
+#pragma unroll
+    for (int k = 0; k < %(cdim)d; ++k) d%(pos)d[(op2_i64)w_t * %(cdim)d + k] = acc[k];
+  }''' % sub)
+        out.append(ind(self._red_finish('b'), 2))
         return out
 
     def _owner_mat_body(self, decls, compute):
         """Owner-computes Mat assembly: a block owns the rows of a range of
-        row-set nodes.  Phase 1 runs the elements naming its nodes and keeps
-        their local matrices (and column-map rows) in LDS; phase 2 gives each
-        owned node's CSR slots to a wavefront, one slot per lane, and sums the
-        node's element contributions to that slot in ascending user element
-        order (the order addto_vector/addto_scalar reach it sequentially)."""
+        row-set nodes.  Its contributions are the (element, row position)
+        pairs of its nodes, listed per node in ascending user element order.
+        Phase 1 runs the elements naming its nodes and keeps, per
+        contribution, the element's local-matrix row for that node in LDS.
+        Phase 2 gives each node pair (row node, column node) of the owned
+        rows to a thread, which adds into the pair's CSR entries the
+        local-matrix entries its scatter program lists, in sequential order
+        (the order addto_vector/addto_scalar reach them)."""
         ind = lambda lines, n: '\n'.join(' ' * n + l for l in lines)
         bs = self.bs
         a = self.owner_columns()[0][0]
         p = a.pos
         e0, e1 = self.extents[0], self.extents[1]
         rd, cd = a.rdim, a.cdim_m
-        nent = e0 * e1 * rd * cd
+        ent = e1 * rd * cd          # doubles kept per contribution
         insert = a.access is WRITE
-        store = []
-        if a.vector:
-            store.append('for (int i0 = 0; i0 < %d; ++i0) for (int i1 = 0; i1 < %d; ++i1) '
-                         'for (int r = 0; r < %d; ++r) for (int c = 0; c < %d; ++c) '
-                         'op2_lds[(((i0 * %d + i1) * %d + r) * %d + c) * pitch + l] = '
-                         'lm%d[i0][i1][r][c];' % (e0, e1, rd, cd, e1, rd, cd, p))
-        else:
-            store.append('for (int i0 = 0; i0 < %d; ++i0) for (int i1 = 0; i1 < %d; ++i1) '
-                         'op2_lds[(i0 * %d + i1) * pitch + l] = lm%d[i0][i1];'
-                         % (e0, e1, e1, p))
-        store.append('for (int j = 0; j < %d; ++j) op2_cmap[j * pitch + l] = '
-                     'mc%d[(op2_i64)e * %d + j];' % (a.carity, p, a.carity))
-        op = '=' if insert else '+='
+        sub = {'bs': bs, 'p': p, 'rd': rd, 'cd': cd, 'nac': a.rarity, 'e0': e0, 'e1': e1,
+               'ent': ent, 'rdcd': rd * cd, 'op': '=' if insert else '+='}
+        val = ('lm%(p)d[i0][i1][r][c]' if a.vector else 'lm%(p)d[i0][i1]') % sub
+        store = ['''for (int i0 = 0; i0 < %(e0)d; ++i0) {
+  const int cx = s_inv[l * %(nac)d + i0];
+  if (cx >= 0)
+    for (int i1 = 0; i1 < %(e1)d; ++i1) for (int r = 0; r < %(rd)d; ++r) for (int c = 0; c < %(cd)d; ++c)
+      s_val[((i1 * %(rd)d + r) * %(cd)d + c) * cpitch + cx] = %(val)s;
+}''' % dict(sub, val=val)]
         out = ['  extern __shared__ double op2_lds[];',
                ind(self._red_init(), 2),
                '''  const int nb = (int)count, bid = blockIdx.x;
   const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
   const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
   const int ex0 = p_exoff[b], nex = p_exoff[b + 1] - ex0;
-  const int pitch = (int)lds_pitch;
-  int* op2_cmap = (int*)(op2_lds + (op2_i64)%(nent)d * pitch);
+  const int t0 = p_t0[b], nn = p_t1[b] - t0, kmax = (int)own_k;
+  const int pitch = (int)lds_pitch, cpitch = (int)own_cpitch;
+  // LDS: per contribution its local-matrix row [%(ent)d][cpitch]; per element
+  // the contribution index of each row position [pitch * %(nac)d] (-1: the
+  // row is not owned here); the owned rows' CSR offsets
+  double* s_val = op2_lds;
+  int* s_inv = (int*)(s_val + (op2_i64)%(ent)d * cpitch);
+  int* s_rp = s_inv + %(nac)d * pitch;
+  (void)kmax;
+  const int c0 = p_tp[t0], ncode = p_tp[t0 + nn] - c0;
+  for (int i = threadIdx.x; i < nex * %(nac)d; i += %(bs)d) s_inv[i] = -1;
+  for (int i = threadIdx.x; i <= nn * %(rd)d; i += %(bs)d) s_rp[i] = rp%(p)d[t0 * %(rd)d + i];
+  __syncthreads();
+  // a contribution code is slot * %(nac)d + row position: it indexes s_inv
+  for (int i = threadIdx.x; i < ncode; i += %(bs)d) s_inv[p_contrib[c0 + i]] = i;
+  __syncthreads();
   for (int l = threadIdx.x; l < nex; l += %(bs)d) {
     const unsigned code = (unsigned)p_exec[ex0 + l];
     const int e = (int)(code & 0x7fffffffu);
     const bool prim = (code >> 31) == 0u;
-    (void)prim;''' % {'nent': nent, 'bs': bs},
+    (void)prim;''' % sub,
                ind(decls + compute + store, 4),
                '  }',
                '  __syncthreads();',
-               '''  // one wavefront per owned node; lanes over the CSR slots of its rows
-  const int g0 = p_tgoff[b], ng = p_tgoff[b + 1] - g0;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int g = wv; g < ng; g += %(nw)d) {
-    const int node = p_tgt[g0 + g];
-    const int cb = p_tp[node], ce = p_tp[node + 1];
-    const int s0 = rp%(p)d[node * %(rd)d], s1 = rp%(p)d[node * %(rd)d + %(rd)d];
-    for (int sl = s0 + lane; sl < s1; sl += 64) {
-      int row = node * %(rd)d;
-      while (row + 1 < node * %(rd)d + %(rd)d && rp%(p)d[row + 1] <= sl) ++row;
-      const int rr2 = row - node * %(rd)d;
-      const int col = ci%(p)d[sl];
-      const int m = col / %(cd)d, cc = col - m * %(cd)d;
-      double acc = d%(p)d[sl];
-      for (int c = cb; c < ce; ++c) {
-        const int code = p_contrib[c];
-        const int slot = code / %(nac)d, i0 = code - slot * %(nac)d;
-        for (int j = 0; j < %(e1)d; ++j) {
-          if (op2_cmap[j * pitch + slot] == m)
-            acc %(op)s op2_lds[(((i0 * %(e1)d + j) * %(rd)d + rr2) * %(cd)d + cc) * pitch + slot];
-        }
-      }
-      d%(p)d[sl] = acc;
+               '''  // one thread per node pair of the owned rows; pair pp of node n starts
+  // at rowptr[n * %(rd)d] / %(rdcd)d
+  const int nrow = nn * %(rd)d;
+  const int pp1 = s_rp[nrow] / %(rdcd)d;
+  for (int pp = s_rp[0] / %(rdcd)d + (int)threadIdx.x; pp < pp1; pp += %(bs)d) {
+    int lo = 0, hi = nn - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_rp[mid * %(rd)d] / %(rdcd)d <= pp) lo = mid; else hi = mid - 1;
     }
-  }''' % {'nw': bs // 64, 'p': p, 'rd': rd, 'cd': cd, 'nac': a.rarity, 'e1': e1, 'op': op},
+    const int kcol = pp - s_rp[lo * %(rd)d] / %(rdcd)d;
+    double acc[%(rd)d][%(cd)d];
+    int sl[%(rd)d];
+#pragma unroll
+    for (int r = 0; r < %(rd)d; ++r) {
+      sl[r] = s_rp[lo * %(rd)d + r] + kcol * %(cd)d;
+#pragma unroll
+      for (int c = 0; c < %(cd)d; ++c) acc[r][c] = d%(p)d[sl[r] + c];
+    }
+    const int q1 = p_poff[pp + 1];
+    for (int qq = p_poff[pp]; qq < q1; ++qq) {
+      const int code = own_wide ? ((const int*)p_prog)[qq] : (int)((const unsigned short*)p_prog)[qq];
+      const int cl = code / %(e1)d, j = code - cl * %(e1)d;
+#pragma unroll
+      for (int r = 0; r < %(rd)d; ++r)
+#pragma unroll
+        for (int c = 0; c < %(cd)d; ++c)
+          acc[r][c] %(op)s s_val[((j * %(rd)d + r) * %(cd)d + c) * cpitch + cl];
+    }
+#pragma unroll
+    for (int r = 0; r < %(rd)d; ++r)
+#pragma unroll
+      for (int c = 0; c < %(cd)d; ++c) d%(p)d[sl[r] + c] = acc[r][c];
+  }''' % sub,
                ind(self._red_finish('b'), 2)]
         return out
+
+    def owner_lds_bytes(self, k, max_exec, max_codes):
+        """LDS bytes of an owner block: its elements' increments (Dat), or
+        (Mat) its contributions' local-matrix rows, its elements'
+        row-position table and its rows' CSR offsets."""
+        a = self.owner_columns()[0][0]
+        if a.kind != 'mat':
+            return max_exec * self.owner_slot_bytes()
+        ent = self.extents[1] * a.rdim * a.cdim_m
+        return 8 * ent * max(max_codes, 1) + 4 * a.rarity * max_exec + 4 * (k * a.rdim + 1)
 
     def _params_decl(self):
         decl = ['op2_i64 start', 'op2_i64 count']
@@ -579,10 +701,14 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                      'const int* __restrict__ p_offset', 'const int* __restrict__ p_nelem',
                      'const int* __restrict__ p_nthrcol', 'const int* __restrict__ p_thrcol']
         elif self.mode == 'owner':
-            decl += ['const int* __restrict__ p_tgoff', 'const int* __restrict__ p_tgt',
+            decl += ['const int* __restrict__ p_t0', 'const int* __restrict__ p_t1',
+                     'const int* __restrict__ p_tgoff', 'const int* __restrict__ p_tgt',
+                     'const int* __restrict__ p_tgcb',
                      'const int* __restrict__ p_exoff', 'const int* __restrict__ p_exec',
                      'const int* __restrict__ p_tp', 'const int* __restrict__ p_contrib',
-                     'op2_i64 lds_pitch']
+                     'op2_i64 lds_pitch', 'op2_i64 own_k', 'op2_i64 own_cpitch',
+                     'const int* __restrict__ p_poff', 'const void* __restrict__ p_prog',
+                     'op2_i64 own_wide']
         for a in self.infos:
             decl += [d for d, _ in a.params()]
         if self.has_mats:
@@ -592,7 +718,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
     def source(self):
         bs = self.bs
         ind = lambda lines, n: '\n'.join(' ' * n + l for l in lines)
-        out = [self.PRELUDE, self._const_decls(),
+        out = [self.PRELUDE, self._const_decls(), self._row_types(),
                '#pragma clang force_cuda_host_device begin',
                'inline',
                self.kernel.code,
@@ -637,32 +763,47 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         return '\n'.join(out)
 
     def _reduce_source(self):
+        """One block of 1024 threads combines the per-block partials: each
+        thread loads 4 partials at a time (the loads overlap), then a
+        wavefront shuffle tree and the 16 wavefront sums in order."""
+        nt, unroll = 1024, 4
         decl = ['op2_i64 nparts']
         for a in self.reductions:
             decl += ['%s* __restrict__ g%d' % (a.ctype, a.pos),
                      'const %s* __restrict__ r%d' % (a.ctype, a.pos)]
-        lines = ['extern "C" __global__ void __launch_bounds__(256) %s(%s) {'
-                 % (self.reduce_entry, ', '.join(decl))]
+        lines = ['extern "C" __global__ void __launch_bounds__(%d) %s(%s) {'
+                 % (nt, self.reduce_entry, ', '.join(decl))]
         for a in self.reductions:
             op = _mode(a.access)
             expr = _REDUCE_OPS[op][0]
-            ident = '(%s)0' % a.ctype if op == 'INC' else None
+            init = '(%s)0' % a.ctype if op == 'INC' else 'g%d[k]' % a.pos
             lines.append('  {')
-            lines.append('    __shared__ %s s%d[4];' % (a.ctype, a.pos))
+            lines.append('    __shared__ %s s%d[%d];' % (a.ctype, a.pos, nt // 64))
             lines.append('    for (int k = 0; k < %d; ++k) {' % a.cdim)
-            init = ident if ident else 'g%d[k]' % a.pos
             lines.append('      %s v = %s;' % (a.ctype, init))
-            lines.append('      for (op2_i64 q = threadIdx.x; q < nparts; q += 256) '
-                         '{ %s a = v, b = r%d[q * %d + k]; v = %s; }' % (a.ctype, a.pos, a.cdim, expr))
+            lines.append('      for (op2_i64 q0 = threadIdx.x; q0 < nparts; q0 += %d) {'
+                         % (nt * unroll))
+            lines.append('        %s x[%d];' % (a.ctype, unroll))
+            lines.append('#pragma unroll')
+            lines.append('        for (int u = 0; u < %d; ++u) {' % unroll)
+            lines.append('          const op2_i64 q = q0 + (op2_i64)u * %d;' % nt)
+            lines.append('          x[u] = q < nparts ? r%d[q * %d + k] : (%s)0;'
+                         % (a.pos, a.cdim, a.ctype))
+            lines.append('        }')
+            lines.append('#pragma unroll')
+            lines.append('        for (int u = 0; u < %d; ++u)' % unroll)
+            lines.append('          if (q0 + (op2_i64)u * %d < nparts) { %s a = v, b = x[u]; v = %s; }'
+                         % (nt, a.ctype, expr))
+            lines.append('      }')
             lines.append('      for (int off = 32; off > 0; off >>= 1) '
                          '{ %s a = v, b = op2_shfl_xor(v, off); v = %s; }' % (a.ctype, expr))
             lines.append('      if ((threadIdx.x & 63) == 0) s%d[threadIdx.x >> 6] = v;' % a.pos)
             lines.append('      __syncthreads();')
             lines.append('      if (threadIdx.x == 0) {')
-            lines.append('        %s a = s%d[0], b = s%d[1]; v = %s;' % (a.ctype, a.pos, a.pos, expr))
-            lines.append('        a = v; b = s%d[2]; v = %s; a = v; b = s%d[3]; v = %s;'
-                         % (a.pos, expr, a.pos, expr))
-            lines.append('        a = g%d[k]; b = v; g%d[k] = %s;' % (a.pos, a.pos, expr))
+            lines.append('        v = s%d[0];' % a.pos)
+            lines.append('        for (int w = 1; w < %d; ++w) { %s a = v, b = s%d[w]; v = %s; }'
+                         % (nt // 64, a.ctype, a.pos, expr))
+            lines.append('        %s a = g%d[k], b = v; g%d[k] = %s;' % (a.ctype, a.pos, a.pos, expr))
             lines.append('      }')
             lines.append('      __syncthreads();')
             lines.append('    }')

@@ -24,6 +24,13 @@ DEFAULTS = {
     'hip_block_size': 0,          # 0: taken from the autotuner table
     'hip_reorder': True,          # locality-ordered partitions for indirect loops
     'hip_autotune': False,
+    'hip_owner_fill': 0.9,        # owner blocks run about this many elements per thread
+    'hip_owner_block_size': 512,  # workgroup size of owner-computes loops (0: hip_block_size)
+    'hip_stage_reads': False,     # stage read-only multi-component direct Dats through LDS
+    'hip_stage_writes': True,     # stage written ones through LDS (else whole-row vector stores)
+    'hip_owner_prefetch': 1,      # owner write-back lookups issued before the elements run (0/1)
+    'hip_layout': True,           # locality renumbering of device storage
+    'hip_layout_order': 'bisect',  # adjacency order of sets no map orders: 'bisect' or 'bfs'
     'hip_cache_dir': None,        # on-disk code-object cache (None: ~/.cache/pyop2_amd)
 }
 

@@ -19,9 +19,12 @@
 // (block, thread colour) share a race target, and no two blocks of one
 // colour share a race target.
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -626,6 +629,266 @@ int op2hip_order_bfs(int ntarget, int nelem, const int* map, int dim, int* perm)
   return OP2HIP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Runs f(thread, lo, hi) over [0, n) split into nt contiguous pieces.
+template <class F>
+void parallel_ranges(int n, int nt, F f) {
+  if (nt <= 1) {
+    f(0, 0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int i = 0; i < nt; ++i) {
+    int lo = static_cast<int>(static_cast<long long>(n) * i / nt);
+    int hi = static_cast<int>(static_cast<long long>(n) * (i + 1) / nt);
+    th.emplace_back(f, i, lo, hi);
+  }
+  for (auto& t : th) t.join();
+}
+
+// Host threads for plan and ordering work: OP2HIP_HOST_THREADS if set, else
+// the hardware's, at most 16; one for small inputs.
+int host_threads(long long work) {
+  if (const char* env = getenv("OP2HIP_HOST_THREADS")) {
+    int v = atoi(env);
+    if (v > 0) return std::min(v, 64);
+  }
+  if (work < (1 << 20)) return 1;
+  unsigned h = std::thread::hardware_concurrency();
+  return static_cast<int>(std::max(1u, std::min(h, 16u)));
+}
+
+// Distinct-neighbour adjacency of the targets of a map (CSR): two targets
+// are adjacent when one element names both.  Map values are checked by the
+// caller.
+struct Adjacency {
+  std::vector<long long> ptr;
+  std::vector<int> adj;
+};
+
+void build_adjacency(int ntarget, int nelem, const int* map, int dim, int nt, Adjacency& g) {
+  std::vector<long long> eptr(ntarget + 1, 0);
+  std::vector<int> eadj(static_cast<size_t>(nelem) * dim);
+  for (long long k = 0; k < static_cast<long long>(nelem) * dim; ++k) eptr[map[k] + 1]++;
+  for (int t = 0; t < ntarget; ++t) eptr[t + 1] += eptr[t];
+  {
+    std::vector<long long> cur(eptr.begin(), eptr.end() - 1);
+    for (int e = 0; e < nelem; ++e)
+      for (int d = 0; d < dim; ++d) eadj[cur[map[static_cast<long long>(e) * dim + d]]++] = e;
+  }
+  // each thread collects the neighbour lists of a contiguous target range
+  std::vector<std::vector<int>> part(nt);
+  std::vector<long long> deg(ntarget + 1, 0);
+  parallel_ranges(ntarget, nt, [&](int th, int lo, int hi) {
+    std::vector<int> seen(ntarget, -1);
+    std::vector<int>& out = part[th];
+    for (int t = lo; t < hi; ++t) {
+      seen[t] = t;
+      size_t before = out.size();
+      for (long long k = eptr[t]; k < eptr[t + 1]; ++k) {
+        const int* row = map + static_cast<long long>(eadj[k]) * dim;
+        for (int d = 0; d < dim; ++d) {
+          int u = row[d];
+          if (seen[u] != t) {
+            seen[u] = t;
+            out.push_back(u);
+          }
+        }
+      }
+      deg[t + 1] = static_cast<long long>(out.size() - before);
+    }
+  });
+  g.ptr.assign(ntarget + 1, 0);
+  for (int t = 0; t < ntarget; ++t) g.ptr[t + 1] = g.ptr[t] + deg[t + 1];
+  g.adj.resize(static_cast<size_t>(g.ptr[ntarget]));
+  parallel_ranges(ntarget, nt, [&](int th, int lo, int hi) {
+    if (lo < hi) std::copy(part[th].begin(), part[th].end(), g.adj.begin() + g.ptr[lo]);
+  });
+}
+
+// Breadth-first order of a whole graph: every component from a
+// pseudo-peripheral vertex (the last reached by a sweep from its lowest
+// vertex), components in order of their lowest vertex.  comp receives the
+// first position of every component and n.
+void bfs_order(const Adjacency& g, int n, std::vector<int>& ord, std::vector<int>& comp) {
+  std::vector<int> mark(n, 0), sweep;
+  ord.clear();
+  ord.reserve(n);
+  int next = 0;
+  auto run = [&](int s, std::vector<int>& out, int stamp) {
+    size_t head = out.size();
+    out.push_back(s);
+    mark[s] = stamp;
+    while (head < out.size()) {
+      int t = out[head++];
+      for (long long k = g.ptr[t]; k < g.ptr[t + 1]; ++k) {
+        int u = g.adj[k];
+        if (mark[u] < stamp) {
+          mark[u] = stamp;
+          out.push_back(u);
+        }
+      }
+    }
+  };
+  comp.clear();
+  while (static_cast<int>(ord.size()) < n) {
+    while (mark[next] == 2) ++next;
+    sweep.clear();
+    run(next, sweep, 1);
+    comp.push_back(static_cast<int>(ord.size()));
+    run(sweep.back(), ord, 2);
+  }
+  comp.push_back(n);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Recursive bisection ordering of the targets of a map (adjacency as for
+// op2hip_order_bfs).  A range of targets is ordered breadth-first from a
+// pseudo-peripheral target of the subgraph it induces (components one after
+// another) and cut at its middle; each half is ordered the same way, down to
+// ranges of at most `leaf` targets, which keep their breadth-first order.
+// Every aligned range of the result is a compact patch of the mesh at every
+// scale (a graph analogue of a space-filling curve), so a block that owns a
+// range of targets shares few elements with its neighbours.  Connected
+// components are ordered one after another (in order of their lowest target)
+// and each exactly as it would be on its own.  The graph is
+// first renumbered breadth-first for cache locality; independent ranges are
+// ordered on up to 16 host threads (the result does not depend on how many).
+int op2hip_order_bisect(int ntarget, int nelem, const int* map, int dim, int leaf, int* perm) {
+  OP2HIP_REQUIRE(ntarget >= 0 && nelem >= 0 && dim > 0 && perm && leaf > 0,
+                 "bad ordering arguments");
+  if (ntarget == 0) return OP2HIP_OK;
+  for (long long k = 0; k < static_cast<long long>(nelem) * dim; ++k)
+    OP2HIP_REQUIRE(map[k] >= 0 && map[k] < ntarget, "map value %d out of range [0,%d)", map[k],
+                   ntarget);
+  const int nt = host_threads(static_cast<long long>(nelem) * dim * dim);
+  Adjacency g0;
+  build_adjacency(ntarget, nelem, map, dim, nt, g0);
+  // renumber breadth-first: vertex v of the working graph is target ord[v]
+  std::vector<int> ord, iord(ntarget), comp;
+  bfs_order(g0, ntarget, ord, comp);
+  for (int v = 0; v < ntarget; ++v) iord[ord[v]] = v;
+  Adjacency g;
+  g.ptr.assign(ntarget + 1, 0);
+  for (int v = 0; v < ntarget; ++v) g.ptr[v + 1] = g.ptr[v] + (g0.ptr[ord[v] + 1] - g0.ptr[ord[v]]);
+  g.adj.resize(g0.adj.size());
+  parallel_ranges(ntarget, nt, [&](int, int lo, int hi) {
+    for (int v = lo; v < hi; ++v) {
+      long long o = g0.ptr[ord[v]], k = g.ptr[v], e = g.ptr[v + 1];
+      for (; k < e; ++k, ++o) g.adj[k] = iord[g0.adj[o]];
+    }
+  });
+  Adjacency().adj.swap(g0.adj);
+
+  const int n = ntarget;
+  // part[v]: the range v currently belongs to, as lo * (n + 1) + hi
+  std::vector<long long> part(n, static_cast<long long>(n));
+  std::vector<int> order(n);
+  std::vector<long long> stamp(n, -1), placed(n, -1);
+  for (int v = 0; v < n; ++v) order[v] = v;
+  struct Range { int lo, hi; };
+  // stamps are unique across threads: thread th uses th, th + nt, ...
+  auto order_range = [&](Range r, int th, long long& counter, std::vector<int>& seq,
+                         std::vector<int>& sweep, std::vector<Range>& out) {
+    const long long pid = static_cast<long long>(r.lo) * (n + 1) + r.hi;
+    const int len = r.hi - r.lo;
+    auto bfs = [&](int s, std::vector<int>& o, long long st) {
+      size_t head = o.size();
+      o.push_back(s);
+      stamp[s] = st;
+      while (head < o.size()) {
+        int t = o[head++];
+        for (long long k = g.ptr[t]; k < g.ptr[t + 1]; ++k) {
+          int u = g.adj[k];
+          if (part[u] == pid && stamp[u] != st) {
+            stamp[u] = st;
+            o.push_back(u);
+          }
+        }
+      }
+    };
+    // breadth-first order of the range, component by component (a
+    // component never holds targets placed before, so sweeps need no check)
+    seq.clear();
+    int scan = r.lo;
+    const long long mark = th + nt * counter++;
+    while (static_cast<int>(seq.size()) < len) {
+      while (placed[order[scan]] == mark) ++scan;
+      sweep.clear();
+      bfs(order[scan], sweep, th + nt * counter++);
+      const size_t b = seq.size();
+      bfs(sweep.back(), seq, th + nt * counter++);
+      for (size_t k = b; k < seq.size(); ++k) placed[seq[k]] = mark;
+    }
+    std::copy(seq.begin(), seq.end(), order.begin() + r.lo);
+    if (len <= leaf) return;
+    const int mid = r.lo + len / 2;
+    const long long left = static_cast<long long>(r.lo) * (n + 1) + mid;
+    const long long right = static_cast<long long>(mid) * (n + 1) + r.hi;
+    for (int k = r.lo; k < mid; ++k) part[order[k]] = left;
+    for (int k = mid; k < r.hi; ++k) part[order[k]] = right;
+    out.push_back({mid, r.hi});
+    out.push_back({r.lo, mid});
+  };
+  // Components are bisected separately (a component is ordered as it
+  // would be alone).  The working numbering is already a breadth-first
+  // order of each component from a pseudo-peripheral vertex: it is the
+  // first level's order.
+  std::vector<Range> level, next;
+  for (size_t c = 0; c + 1 < comp.size(); ++c) {
+    const int lo = comp[c], hi = comp[c + 1];
+    if (hi - lo <= leaf) continue;
+    const int mid = lo + (hi - lo) / 2;
+    const long long left = static_cast<long long>(lo) * (n + 1) + mid;
+    const long long right = static_cast<long long>(mid) * (n + 1) + hi;
+    for (int k = lo; k < mid; ++k) part[k] = left;
+    for (int k = mid; k < hi; ++k) part[k] = right;
+    level.push_back({lo, mid});
+    level.push_back({mid, hi});
+  }
+  // further levels range by range on the threads, until every thread has
+  // several ranges; then each range with its whole subtree on one thread
+  std::vector<long long> counters(nt, 0);
+  std::vector<std::vector<int>> seqs(nt), sweeps(nt);
+  std::vector<std::vector<Range>> outs(nt);
+  while (!level.empty() && static_cast<int>(level.size()) < 4 * nt) {
+    std::atomic<int> cursor(0);
+    const int nr = static_cast<int>(level.size());
+    std::vector<std::vector<Range>> kids(nr);
+    parallel_ranges(std::min(nt, nr), std::min(nt, nr), [&](int th, int, int) {
+      for (int i = cursor++; i < nr; i = cursor++)
+        order_range(level[i], th, counters[th], seqs[th], sweeps[th], kids[i]);
+    });
+    next.clear();
+    for (int i = 0; i < nr; ++i)
+      for (auto it = kids[i].rbegin(); it != kids[i].rend(); ++it) next.push_back(*it);
+    level.swap(next);
+  }
+  std::sort(level.begin(), level.end(),
+            [](const Range& a, const Range& b) { return a.hi - a.lo > b.hi - b.lo; });
+  std::atomic<int> cursor(0);
+  const int nr = static_cast<int>(level.size());
+  parallel_ranges(nt, nt, [&](int th, int, int) {
+    std::vector<Range> stack;
+    for (int i = cursor++; i < nr; i = cursor++) {
+      stack.assign(1, level[i]);
+      while (!stack.empty()) {
+        Range r = stack.back();
+        stack.pop_back();
+        order_range(r, th, counters[th], seqs[th], sweeps[th], stack);
+      }
+    }
+  });
+  for (int pos = 0; pos < n; ++pos) perm[pos] = ord[order[pos]];
+  return OP2HIP_OK;
+}
+
 // First-touch ordering of the targets of a map from an ordered source set:
 // walking the source elements in `src_order`, each target takes the next
 // position when first named; targets never named follow in index order.
@@ -648,6 +911,100 @@ int op2hip_order_first_touch(int ntarget, int nsrc, const int* map, int dim,
   }
   for (int t = 0; t < ntarget; ++t)
     if (!seen[t]) perm[k++] = t;
+  return OP2HIP_OK;
+}
+
+}  // extern "C"
+
+// ---- owner Mat assembly program ---------------------------------------------
+
+extern "C" {
+
+// Scatter program of an owner-computes Mat assembly: for every node pair
+// (row node n, column node m) of the Mat's node-level pattern, in pattern
+// order, the list of (block-local contribution index c, column position j)
+// whose local-matrix entry [i0][j] adds into that pair, in the order the
+// sequential backend adds them (ascending user element, then j).  A
+// contribution is an (element, row position i0) pair of an owned row node;
+// the owner plan lists them per node in ascending user element order.
+//
+// The node-level pattern comes from the scalar CSR (rowptr/colidx, device
+// numbering) with rd x cd blocks: node n's pairs are the columns of scalar
+// row n*rd taken every cd-th entry, and pair p of n starts at
+// rowptr[n*rd] / (rd*cd).  Outputs (caller-allocated): poff[npairs+1]
+// (npairs = rowptr[nnodes*rd] / (rd*cd)) and prog[ncontrib * e1] holding
+// c * e1 + j as uint16 (wide == 0) or int32 (wide == 1).  Fails if an
+// element names a column the pattern lacks or a code does not fit.
+int op2hip_mat_program(int nactive, const int* blk_t0, const int* blk_t1, const int* ex_off,
+                       const int* exec, const int* tp, const int* contrib, int nac,
+                       const int* cmap, int carity, int e1, int nnodes, const int* rowptr,
+                       const int* colidx, int rd, int cd, int wide, int* poff, void* prog) {
+  OP2HIP_REQUIRE(nactive >= 0 && nac > 0 && carity > 0 && e1 > 0 && e1 <= carity && rd > 0 &&
+                     cd > 0 && poff && prog,
+                 "bad Mat program arguments");
+  const long long npairs = static_cast<long long>(rowptr[static_cast<long long>(nnodes) * rd]) /
+                           (static_cast<long long>(rd) * cd);
+  // entries per pair: each contribution of node n adds e1 entries to n's pairs
+  std::vector<int> cnt(static_cast<size_t>(npairs) + 1, 0);
+  std::atomic<int> bad(0);
+  const int nt = host_threads(static_cast<long long>(tp[nnodes]) * e1);
+  auto pair_of = [&](int n, int m) -> long long {
+    // columns of scalar row n*rd, one per cd: binary search for m
+    const int s0 = rowptr[static_cast<long long>(n) * rd], s1 = rowptr[static_cast<long long>(n) * rd + 1];
+    int lo = 0, hi = (s1 - s0) / cd - 1;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int v = colidx[s0 + mid * cd] / cd;
+      if (v == m) return static_cast<long long>(s0) / (static_cast<long long>(rd) * cd) + mid;
+      if (v < m) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+  };
+  // pass 1: count (pairs of a node are touched only by its own block)
+  parallel_ranges(nactive, nt, [&](int, int b0, int b1) {
+    for (int b = b0; b < b1; ++b)
+      for (int n = blk_t0[b]; n < blk_t1[b]; ++n)
+        for (int c = tp[n]; c < tp[n + 1]; ++c) {
+          const int slot = contrib[c] / nac;
+          const int e = static_cast<int>(static_cast<unsigned>(exec[ex_off[b] + slot]) & 0x7fffffffu);
+          for (int j = 0; j < e1; ++j) {
+            const long long p = pair_of(n, cmap[static_cast<long long>(e) * carity + j]);
+            if (p < 0) { bad = 1; continue; }
+            cnt[p + 1]++;
+          }
+        }
+  });
+  OP2HIP_REQUIRE(!bad, "an element names a matrix entry outside the sparsity pattern");
+  for (long long p = 0; p < npairs; ++p) cnt[p + 1] += cnt[p];
+  OP2HIP_REQUIRE(static_cast<long long>(cnt[npairs]) ==
+                     static_cast<long long>(tp[nnodes]) * e1 || nactive == 0,
+                 "Mat program size mismatch");
+  for (long long p = 0; p <= npairs; ++p) poff[p] = cnt[p];
+  // pass 2: fill, contributions of each node in their (element) order
+  parallel_ranges(nactive, nt, [&](int, int b0, int b1) {
+    std::vector<int> cur;
+    for (int b = b0; b < b1; ++b) {
+      const int cbase = tp[blk_t0[b]];
+      for (int n = blk_t0[b]; n < blk_t1[b]; ++n)
+        for (int c = tp[n]; c < tp[n + 1]; ++c) {
+          const int slot = contrib[c] / nac;
+          const int e = static_cast<int>(static_cast<unsigned>(exec[ex_off[b] + slot]) & 0x7fffffffu);
+          const long long code0 = static_cast<long long>(c - cbase) * e1;
+          for (int j = 0; j < e1; ++j) {
+            const long long p = pair_of(n, cmap[static_cast<long long>(e) * carity + j]);
+            const long long code = code0 + j;
+            const int at = cnt[p]++;
+            if (wide) {
+              static_cast<int*>(prog)[at] = static_cast<int>(code);
+            } else {
+              if (code > 0xffff) bad = 1;
+              static_cast<uint16_t*>(prog)[at] = static_cast<uint16_t>(code);
+            }
+          }
+        }
+    }
+  });
+  OP2HIP_REQUIRE(!bad, "Mat program codes do not fit 16 bits");
   return OP2HIP_OK;
 }
 

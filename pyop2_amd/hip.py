@@ -554,7 +554,7 @@ class OwnerPlan(object):
     elements naming its targets and sums each owned target's increments in
     ascending element order."""
 
-    ARRAYS = ('blk_t0', 'blk_t1', 'ex_off', 'exec', 'tp', 'contrib', 'tg_off', 'tgt')
+    ARRAYS = ('blk_t0', 'blk_t1', 'ex_off', 'exec', 'tp', 'contrib', 'tg_off', 'tgt', 'tg_cb')
 
     def __init__(self, n, ntargets, cols, k, elem_order=None):
         self.n, self.ntargets, self.k = n, ntargets, k
@@ -576,7 +576,7 @@ class OwnerPlan(object):
             sizes = {'blk_t0': self.nactive, 'blk_t1': self.nactive,
                      'ex_off': self.nactive + 1, 'exec': nexec, 'tp': ntargets + 1,
                      'contrib': ncontrib, 'tg_off': self.nactive + 1, 'tgt': ntgt}
-            for w, name in enumerate(self.ARRAYS):
+            for w, name in enumerate(self.ARRAYS[:8]):
                 arr = np.zeros(sizes[name], dtype=np.int32)
                 if arr.size:
                     nat.check(nat.lib.op2hip_owner_plan_array(
@@ -584,6 +584,15 @@ class OwnerPlan(object):
                 setattr(self, name, arr)
         finally:
             nat.lib.op2hip_owner_plan_destroy(h)
+        # contribution offsets by touched-target position (targets without
+        # increments have empty ranges, so the next entry ends each range)
+        self.tg_cb = np.append(self.tp[self.tgt], self.tp[-1]).astype(np.int32)
+        # contribution codes per block (its owned targets' range of contrib)
+        self.max_codes = int((self.tp[self.blk_t1] - self.tp[self.blk_t0]).max()) \
+            if self.nactive else 0
+        # device lists carry one padding entry (tgt 0, tg_cb its end), so a
+        # kernel may load entry ng of a block unconditionally
+        self._pad = {'tgt': 0, 'tg_cb': int(self.tp[-1])}
         self._dev = None
 
     def device_arrays(self):
@@ -591,6 +600,8 @@ class OwnerPlan(object):
             self._dev = {}
             for name in self.ARRAYS:
                 a = getattr(self, name)
+                if name in self._pad:
+                    a = np.append(a, np.int32(self._pad[name])).astype(np.int32)
                 d = _dev_bytes(a.nbytes)
                 _h2d(d, a)
                 self._dev[name] = d
@@ -599,6 +610,38 @@ class OwnerPlan(object):
 
 _owner_cache = {}
 OWNER_LDS_LIMIT = 64 * 1024
+
+
+def _mat_program(plan, arg, e1):
+    """Device scatter program of an owner Mat assembly (csrc/plan.cpp
+    op2hip_mat_program), cached on the plan per sparsity: (poff, prog,
+    wide)."""
+    sp = arg.data.sparsity
+    progs = plan.__dict__.setdefault('_progs', {})
+    hit = progs.get(id(sp))
+    if hit is not None and hit[0] is sp:
+        return hit[1]
+    rd, cd = arg.data.dims
+    rp = sp._dev_rowptr.cpu().numpy().view(np.int32)
+    ci = sp._dev_colidx.cpu().numpy().view(np.int32)
+    cmap = np.ascontiguousarray(arg.map[1]._layout_values(), dtype=np.int32)
+    nnodes = arg.map[0].dataset.total_size
+    npairs = int(rp[nnodes * rd]) // (rd * cd)
+    wide = int(max(plan.max_codes, 1) * e1 > 0xffff)
+    poff = np.empty(npairs + 1, dtype=np.int32)
+    prog = np.empty(len(plan.contrib) * e1, dtype=np.int32 if wide else np.uint16)
+    arrs = [np.ascontiguousarray(x, dtype=np.int32) for x in
+            (plan.blk_t0, plan.blk_t1, plan.ex_off, plan.exec, plan.tp, plan.contrib)]
+    P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    nat.check(nat.lib.op2hip_mat_program(
+        plan.nactive, *[P(x) for x in arrs], plan.nac, P(cmap), cmap.shape[1], e1, nnodes,
+        P(rp), P(ci), rd, cd, wide, P(poff), P(prog)), 'Mat scatter program')
+    d_poff, d_prog = _dev_bytes(poff.nbytes), _dev_bytes(prog.nbytes)
+    _h2d(d_poff, poff)
+    _h2d(d_prog, prog)
+    res = (d_poff, d_prog, wide)
+    progs[id(sp)] = (sp, res)
+    return res
 
 
 def _owner_columns(args):
@@ -614,11 +657,13 @@ def _owner_columns(args):
     return cols
 
 
-def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256):
+def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None):
     """Cached owner plan.  The owned range ``k`` is chosen so a block runs
     about one element per thread (a first build measures elements run per
-    owned target), and halved until a block's increments fit
-    ``OWNER_LDS_LIMIT`` bytes of LDS (``slot_bytes`` per element)."""
+    owned target), and shrunk until a block's LDS (``lds_of(plan)``; by
+    default ``slot_bytes`` per element) fits ``OWNER_LDS_LIMIT`` bytes."""
+    if lds_of is None:
+        lds_of = lambda p: p.max_exec * slot_bytes
     cols = _owner_columns(args)
     target = cols[0][0].dataset
     k_cfg = k or cfg.get('hip_owner_targets')
@@ -642,14 +687,17 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256):
         kk = 128
         p = build(kk)
         if p.nactive:
+            # elements run and LDS bytes per owned target, from the trial
             per_target = (p.ex_off[-1] / float(p.nactive)) / kk
-            kk = min(1 << 16, max(1, int(0.9 * block / max(per_target, 1e-9))))
-            kk = max(1, min(kk, int(limit / slot_bytes / max(per_target, 1e-9))))
+            lds_per_target = lds_of(p) / float(kk)
+            fill = cfg.get('hip_owner_fill') or 0.9
+            kk = min(1 << 16, max(1, int(fill * block / max(per_target, 1e-9))))
+            kk = max(1, min(kk, int(limit / max(lds_per_target, 1e-9))))
             p = build(kk)
-    while p.max_exec * slot_bytes > limit and kk > 1:
+    while lds_of(p) > limit and kk > 1:
         kk = max(1, int(kk * 0.8))
         p = build(kk)
-    if p.max_exec * slot_bytes > limit:
+    if lds_of(p) > limit:
         _owner_cache[key] = None
         return None
     if p.nactive and p.ex_off[-1] / float(p.nactive) < 32 and not k_cfg:
@@ -727,11 +775,13 @@ class _LaunchEntry(object):
         lds = 0
         if mode == 'owner':
             slot_bytes = loop.source.owner_slot_bytes()
-            plan = get_owner_plan(iterset, n, args, slot_bytes, block=bs)
+            src = loop.source
+            plan = get_owner_plan(iterset, n, args, slot_bytes, block=bs,
+                                  lds_of=lambda p: src.owner_lds_bytes(p.k, p.max_exec, p.max_codes))
             if plan is None:
                 raise _NoOwnerPlan()
             nparts = max(plan.nactive, 1)
-            lds = plan.max_exec * slot_bytes
+            lds = src.owner_lds_bytes(plan.k, plan.max_exec, plan.max_codes)
             grid_stream = 0
         elif mode == 'coloured':
             plan = get_plan(pl._it_space.iterset, n, args,
@@ -780,10 +830,18 @@ class _LaunchEntry(object):
             elif mode == 'owner':
                 pdev = plan.device_arrays()
                 self.keep.append(plan)
-                names = {'own_tgoff': 'tg_off', 'own_tgt': 'tgt', 'own_exoff': 'ex_off',
+                prog = (None, None, 0)
+                if any(x._is_mat for x in args):
+                    ma = [x for x in args if x._is_mat][0]
+                    prog = _mat_program(plan, ma, pl._it_space.extents[1])
+                names = {'own_t0': 'blk_t0', 'own_t1': 'blk_t1', 'own_tgcb': 'tg_cb',
+                         'own_tgoff': 'tg_off', 'own_tgt': 'tgt', 'own_exoff': 'ex_off',
                          'own_exec': 'exec', 'own_tp': 'tp', 'own_contrib': 'contrib'}
                 argv = [0 if r == 'start' else plan.nactive if r == 'count' else
-                        plan.max_exec if r == 'own_pitch' else
+                        plan.max_exec if r == 'own_pitch' else plan.k if r == 'own_k' else
+                        max(plan.max_codes, 1) if r == 'own_cpitch' else
+                        _ptr(prog[0]) if r == 'own_poff' else _ptr(prog[1]) if r == 'own_prog' else
+                        prog[2] if r == 'own_wide' else
                         _ptr(pdev[names[r]]) if isinstance(r, str) else r for r in vals]
                 if plan.nactive:
                     launches.append((loop.fn, plan.nactive, bs, _argv(argv), lds))
@@ -817,7 +875,7 @@ class _LaunchEntry(object):
                         rargv.append(_ptr(args[pos].data._dev))
                     else:
                         rargv.append(_ptr(partials[pos]))
-                launches.append((loop.reduce_fn, 1, 256, _argv(rargv), 0))
+                launches.append((loop.reduce_fn, 1, 1024, _argv(rargv), 0))
         self.launches = launches
         self.written = [a.data for a in args if (a._is_dat and a.access is not READ) or
                         a._is_global_reduction]
@@ -917,17 +975,22 @@ class ParLoop(base.ParLoop):
 
     def _compiled(self, mode, bs, consts):
         key = base.JITModule._cache_key(self.kernel, self._it_space.extents, *self.args)
-        key = key + (mode, bs)
+        pf = int(cfg.get('hip_owner_prefetch', 1))
+        sr = bool(cfg.get('hip_stage_reads', False))
+        sw = bool(cfg.get('hip_stage_writes', True))
+        key = key + (mode, bs, pf, sr, sw)
         c = _loop_cache.get(key)
         if c is None:
-            src = LoopSource(self.kernel, self._it_space.extents, self.args, consts, mode, bs)
+            src = LoopSource(self.kernel, self._it_space.extents, self.args, consts, mode, bs,
+                             prefetch=pf, stage_reads=sr, stage_writes=sw)
             c = _CompiledLoop(src)
             _loop_cache[key] = c
         return c
 
     def compute(self):
         bs = _block_size()
-        key = (id(self.kernel), self._it_space.extents, id(self._it_space.iterset), bs,
+        obs = cfg.get('hip_owner_block_size') or bs
+        key = (id(self.kernel), self._it_space.extents, id(self._it_space.iterset), bs, obs,
                cfg.get('hip_partition_size'), cfg.get('hip_reorder', True),
                tuple(_arg_key(a) for a in self.args),
                tuple((c.name, id(c)) for c in base.Const._definitions()))
@@ -935,8 +998,9 @@ class ParLoop(base.ParLoop):
         if ent is None:
             if len(_call_cache) >= _CALL_CACHE_MAX:
                 _call_cache.pop(next(iter(_call_cache)))
+            mode = self._mode()
             try:
-                ent = _LaunchEntry(self, self._mode(), bs)
+                ent = _LaunchEntry(self, mode, obs if mode == 'owner' else bs)
             except _NoOwnerPlan:
                 ent = _LaunchEntry(self, 'coloured', bs)
             _call_cache[key] = ent

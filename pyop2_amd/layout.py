@@ -10,8 +10,10 @@ chosen for locality, decided once per Set at its first device use:
    position its elements name there (edges follow their cells);
 2. a Set that an already laid-out Set maps into takes the order in which
    those elements first name it (nodes follow the cells that use them);
-3. otherwise a breadth-first order of the Set's adjacency through the
-   largest map into it (cells adjacent across edges);
+3. otherwise a recursive bisection of the Set's adjacency through the
+   largest map into it (cells adjacent across edges): breadth-first cuts in
+   halves down to ranges of 32, so every aligned range of positions is a
+   compact patch of the mesh;
 4. otherwise the identity.
 
 Sets used directly in one loop (a Dat on ``vcells`` read in a loop over
@@ -150,15 +152,11 @@ def _decide(g):
         if _group(U) is not g and decided(U) and not of(U).identity:
             src = np.ascontiguousarray(of(U).perm, dtype=np.int32)
             return Layout(_first_touch(n, m, src))
-    # 3. breadth-first order through the largest map into the set
+    # 3. recursive bisection (or breadth-first order) of the adjacency
+    #    through the largest map into the set
     if maps_to:
         m = max(maps_to, key=lambda x: x.values.size)
-        v = np.ascontiguousarray(m.values, dtype=np.int32)
-        perm = np.empty(n, dtype=np.int32)
-        nat.check(nat.lib.op2hip_order_bfs(n, v.shape[0], v.ctypes.data_as(ctypes.c_void_p),
-                                           v.shape[1], perm.ctypes.data_as(ctypes.c_void_p)),
-                  'bfs ordering')
-        return Layout(perm)
+        return Layout(adjacency_order(n, m.values, cfg.get('hip_layout_order', 'bisect')))
     # 4. an iteration set only: lay out its largest target first, then follow it
     if maps_from:
         m = max(maps_from, key=lambda x: x.dataset.total_size)
@@ -168,6 +166,20 @@ def _decide(g):
                 key = T.iperm[m.values].min(axis=1)
                 return Layout(np.argsort(key, kind='stable'))
     return IDENTITY
+
+
+def adjacency_order(n, values, how='bisect', leaf=32):
+    """perm of ``n`` targets ordered for locality through map ``values``
+    (nelem x arity): 'bisect' (recursive breadth-first bisection) or 'bfs'."""
+    v = np.ascontiguousarray(values, dtype=np.int32)
+    perm = np.empty(n, dtype=np.int32)
+    vp, pp = v.ctypes.data_as(ctypes.c_void_p), perm.ctypes.data_as(ctypes.c_void_p)
+    if how == 'bfs':
+        nat.check(nat.lib.op2hip_order_bfs(n, v.shape[0], vp, v.shape[1], pp), 'bfs ordering')
+    else:
+        nat.check(nat.lib.op2hip_order_bisect(n, v.shape[0], vp, v.shape[1], leaf, pp),
+                  'bisection ordering')
+    return perm
 
 
 def _first_touch(n, m, src_order):

@@ -23,10 +23,11 @@ def airfoil_sets(mesh):
     return nodes, edges, cells, vcells, pedge, pecell, pevcell, pcell
 
 
-def spread(vals, npos):
-    """Mean distance between the first and other columns of map rows."""
-    v = vals.astype(np.int64)
-    return np.abs(v[:, 1:] - v[:, :1]).mean()
+def same_block(vals, b):
+    """Fraction of map rows whose entries all fall in one aligned range of
+    ``b`` positions."""
+    v = vals.astype(np.int64) // b
+    return (v == v[:, :1]).all(axis=1).mean()
 
 
 def test_layout_permutations_and_translation():
@@ -45,8 +46,46 @@ def test_layout_permutations_and_translation():
         # device row i is user element perm[i]; its entries are device
         # positions of the user targets
         assert np.array_equal(dst.perm[dev], m.values[src.perm])
-    # locality: neighbouring cells of an edge are close in device numbering
-    assert spread(layout.map_values(pecell), mesh.ncell) < 0.05 * spread(mesh.pecell, mesh.ncell)
+    # locality: the two cells of most edges share an aligned range of 32
+    # device positions (a block that owns the range runs the edge once)
+    dev = same_block(layout.map_values(pecell), 32)
+    assert dev > 0.5 and dev > 10 * same_block(mesh.pecell, 32)
+
+
+@pytest.mark.parametrize('how', ['bisect', 'bfs'])
+def test_adjacency_order_is_a_permutation(how):
+    mesh = meshes.TriMesh(20, seed=3)
+    perm = layout.adjacency_order(mesh.nvert, mesh.elem_node, how)
+    assert sorted(perm.tolist()) == list(range(mesh.nvert))
+    # targets no element names still get positions (isolated components)
+    extra = layout.adjacency_order(mesh.nvert + 7, mesh.elem_node, how)
+    assert sorted(extra.tolist()) == list(range(mesh.nvert + 7))
+
+
+def test_bisect_order_compact_ranges():
+    """Aligned ranges of the bisection order are compact patches: on a
+    1200 x 600 quad mesh an owner block of 128 cells runs far fewer
+    edges twice than with a breadth-first order."""
+    mesh = meshes.QuadMesh(400, 200, seed=5)
+
+    def shared_edges(how, b):
+        perm = layout.adjacency_order(mesh.ncell, mesh.pecell, how)
+        iperm = np.empty_like(perm)
+        iperm[perm] = np.arange(len(perm))
+        return 1.0 - same_block(iperm[mesh.pecell], b)
+
+    for b in (64, 128, 256):
+        assert shared_edges('bisect', b) < 0.6 * shared_edges('bfs', b)
+
+
+def test_bisect_order_independent_of_threads(monkeypatch):
+    """The ordering is the same whatever the host thread count."""
+    mesh = meshes.TetMesh(24, seed=2)
+    out = []
+    for nt in ('1', '3', '8'):
+        monkeypatch.setenv('OP2HIP_HOST_THREADS', nt)
+        out.append(layout.adjacency_order(mesh.nvert, mesh.elem_node, 'bisect'))
+    assert all(np.array_equal(out[0], o) for o in out[1:])
 
 
 @pytest.mark.parametrize('dims', [(1, 1), (2, 2)])
@@ -70,3 +109,17 @@ def test_csr_translation_bit_exact(dims):
     assert np.array_equal(ci, sp_user._colidx)
     assert np.array_equal(nnz, sp_user._d_nnz)
     assert sorted(slot.tolist()) == list(range(len(ci)))
+
+
+@pytest.mark.parametrize('how', ['bisect', 'bfs'])
+def test_components_ordered_as_alone(how):
+    """Disconnected meshes in one Set (a batch of meshes) keep contiguous
+    position ranges, each ordered exactly as that mesh alone: the batched
+    solvers rely on it for alone-vs-batch bit identity."""
+    ms = [meshes.TriMesh(n, seed=s) for n, s in ((9, 1), (23, 2), (4, 3), (40, 4))]
+    off = np.cumsum([0] + [m.nvert for m in ms])
+    union = np.concatenate([m.elem_node + o for m, o in zip(ms, off[:-1])])
+    perm = layout.adjacency_order(int(off[-1]), union, how)
+    for k, m in enumerate(ms):
+        alone = layout.adjacency_order(m.nvert, m.elem_node, how)
+        assert np.array_equal(perm[off[k]:off[k + 1]] - off[k], alone)
