@@ -612,20 +612,15 @@ _owner_cache = {}
 OWNER_LDS_LIMIT = 64 * 1024
 
 
-def _mat_program(plan, arg, e1):
-    """Device scatter program of an owner Mat assembly (csrc/plan.cpp
-    op2hip_mat_program), cached on the plan per sparsity: (poff, prog,
-    wide)."""
-    sp = arg.data.sparsity
-    progs = plan.__dict__.setdefault('_progs', {})
-    hit = progs.get(id(sp))
-    if hit is not None and hit[0] is sp:
-        return hit[1]
-    rd, cd = arg.data.dims
-    rp = sp._dev_rowptr.cpu().numpy().view(np.int32)
-    ci = sp._dev_colidx.cpu().numpy().view(np.int32)
-    cmap = np.ascontiguousarray(arg.map[1]._layout_values(), dtype=np.int32)
-    nnodes = arg.map[0].dataset.total_size
+def mat_program_host(plan, cmap, e1, nnodes, rowptr, colidx, rd, cd):
+    """Host arrays (poff, prog, wide) of the scatter program of an owner Mat
+    assembly (csrc/plan.cpp op2hip_mat_program): per node pair of the
+    pattern (rowptr/colidx: the scalar CSR in device numbering), the codes
+    c * e1 + j of the block-local contributions and column positions that
+    add into it, in sequential order."""
+    rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+    ci = np.ascontiguousarray(colidx, dtype=np.int32)
+    cmap = np.ascontiguousarray(cmap, dtype=np.int32)
     npairs = int(rp[nnodes * rd]) // (rd * cd)
     wide = int(max(plan.max_codes, 1) * e1 > 0xffff)
     poff = np.empty(npairs + 1, dtype=np.int32)
@@ -636,6 +631,22 @@ def _mat_program(plan, arg, e1):
     nat.check(nat.lib.op2hip_mat_program(
         plan.nactive, *[P(x) for x in arrs], plan.nac, P(cmap), cmap.shape[1], e1, nnodes,
         P(rp), P(ci), rd, cd, wide, P(poff), P(prog)), 'Mat scatter program')
+    return poff, prog, wide
+
+
+def _mat_program(plan, arg, e1):
+    """Device scatter program of an owner Mat assembly, cached on the plan
+    per sparsity: (poff, prog, wide)."""
+    sp = arg.data.sparsity
+    progs = plan.__dict__.setdefault('_progs', {})
+    hit = progs.get(id(sp))
+    if hit is not None and hit[0] is sp:
+        return hit[1]
+    rd, cd = arg.data.dims
+    poff, prog, wide = mat_program_host(
+        plan, arg.map[1]._layout_values(), e1, arg.map[0].dataset.total_size,
+        sp._dev_rowptr.cpu().numpy().view(np.int32), sp._dev_colidx.cpu().numpy().view(np.int32),
+        rd, cd)
     d_poff, d_prog = _dev_bytes(poff.nbytes), _dev_bytes(prog.nbytes)
     _h2d(d_poff, poff)
     _h2d(d_prog, prog)

@@ -154,3 +154,58 @@ def test_owner_airfoil_edges():
     cols = [(m.pecell, 0), (m.pecell, 1)]
     p = OwnerPlan(m.nedge, m.ncell, [(v, 2, i) for v, i in cols], 64)
     check_owner(p, cols, m.nedge, m.ncell)
+
+
+def _node_csr(elem_node, nnodes, rd, cd):
+    """Scalar CSR of the node pattern of one map pair, expanded rd x cd
+    (sorted unique columns), by brute force."""
+    nbrs = [set() for _ in range(nnodes)]
+    for row in elem_node:
+        for a in row:
+            nbrs[a].update(int(b) for b in row)
+    rows, cols = [0], []
+    for n in range(nnodes):
+        cs = sorted(nbrs[n])
+        for r in range(rd):
+            cols += [cd * m + c for m in cs for c in range(cd)]
+            rows.append(len(cols))
+    return np.array(rows, dtype=np.int32), np.array(cols, dtype=np.int32)
+
+
+@pytest.mark.parametrize('rd,cd,k', [(1, 1, 5), (2, 2, 7), (1, 1, 1)])
+def test_mat_scatter_program(rd, cd, k):
+    """The owner Mat scatter program lists, for every node pair, exactly the
+    (element, row position, column position) entries that add into it, in
+    ascending user element order then column position, and every entry of
+    every element once (brute force on a permuted triangle mesh)."""
+    from pyop2_amd.hip import OwnerPlan, mat_program_host
+    mesh = meshes.TriMesh(7, seed=11)
+    en = mesh.elem_node.astype(np.int32)
+    n, nn = en.shape[0], mesh.nvert
+    order = np.random.default_rng(2).permutation(n).astype(np.int32)   # user order
+    p = OwnerPlan(n, nn, [(en, 3, -1)], k, order)
+    rp, ci = _node_csr(en, nn, rd, cd)
+    poff, prog, wide = mat_program_host(p, en, 3, nn, rp, ci, rd, cd)
+    assert not wide
+    rank = np.empty(n, dtype=np.int64)
+    rank[order] = np.arange(n)
+    seen = set()
+    npairs = len(poff) - 1
+    assert npairs == rp[nn * rd] // (rd * cd)
+    for b in range(p.nactive):
+        cbase = p.tp[p.blk_t0[b]]
+        for node in range(p.blk_t0[b], p.blk_t1[b]):
+            s0 = rp[node * rd]
+            cols = ci[s0:rp[node * rd + 1]:cd] // cd
+            for kk, m in enumerate(cols):
+                pp = s0 // (rd * cd) + kk
+                got = []
+                for code in prog[poff[pp]:poff[pp + 1]]:
+                    c, j = divmod(int(code), 3)
+                    slot, i0 = divmod(int(p.contrib[cbase + c]), p.nac)
+                    e = int(p.exec[p.ex_off[b] + slot]) & 0x7fffffff
+                    assert en[e, i0] == node and en[e, j] == m
+                    got.append((rank[e], j))
+                    seen.add((e, i0, j))
+                assert got == sorted(got)
+    assert len(seen) == n * 9
