@@ -42,6 +42,10 @@ def parse():
     p.add_argument('--partition-size', type=int, default=0)
     p.add_argument('--block-size', type=int, default=0)
     p.add_argument('--owner-block-size', type=int, default=0)
+    p.add_argument('--stream-unroll', type=int, default=0,
+                   help='elements per thread of direct loops (0: the default)')
+    p.add_argument('--stream-blocks', type=int, default=0,
+                   help='most workgroups of a direct loop (0: the default)')
     p.add_argument('--layout-order', default='', help="'bisect' or 'bfs' (default: the backend's)")
     p.add_argument('--stage-reads', type=int, default=-1,
                    help='stage read-only multi-component direct Dats through LDS (-1: default)')
@@ -97,6 +101,10 @@ def main():
         opts['hip_stage_writes'] = bool(args.stage_writes)
     if args.layout_order:
         opts['hip_layout_order'] = args.layout_order
+    if args.stream_unroll:
+        opts['hip_stream_unroll'] = args.stream_unroll
+    if args.stream_blocks:
+        opts['hip_stream_blocks'] = args.stream_blocks
     if args.owner_block_size:
         opts['hip_owner_block_size'] = args.owner_block_size
     if args.owner_targets:

@@ -147,13 +147,15 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
 '''
 
     def __init__(self, kernel, extents, args, consts, mode, block_size, prefetch=1,
-                 stage_reads=True, stage_writes=True):
+                 stage_reads=True, stage_writes=True, unroll=1):
         assert mode in ('stream', 'coloured', 'owner')
         # owner write-back: 1 issues the loads locating a thread's first
         # target before the elements run (they overlap), 0 after them
         self.prefetch = prefetch
         self.stage_reads = stage_reads
         self.stage_writes = stage_writes
+        # elements per thread and chunk of a direct loop (their loads overlap)
+        self.unroll = unroll
         self.kernel = kernel
         self.extents = tuple(extents)
         self.infos = [ArgInfo(k, a, self.extents) for k, a in enumerate(args)]
@@ -249,82 +251,114 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             lines.append('}')
         return lines
 
-    def _element_body(self):
-        """Code that runs the kernel for element ``e`` (all itspace points),
-        leaving INC increments and Mat entries in private arrays.
+    def _element_body(self, sfx='', ev='e', split=False):
+        """Code that runs the kernel for element ``ev`` (all itspace points),
+        leaving INC increments and Mat entries in private arrays; private
+        names carry the suffix ``sfx``.
 
         Returns (decls, compute): ``decls`` declares and zeroes the private
         arrays and touches no memory, so it can run for idle threads of a
-        workgroup; ``compute`` reads maps and data and calls the kernel."""
-        decls, compute, call_args, zeros, post = [], [], [], [], []
+        workgroup; ``compute`` reads maps and data and calls the kernel.
+        With ``split`` it returns (decls, loads, compute): ``loads`` copies
+        the rows of read-only Dats into registers (local rows), so the
+        template can issue the loads of several elements before their
+        kernels run."""
+        decls, loads, compute, call_args, zeros, post = [], [], [], [], [], []
         ext = self.extents
+        written = set(id(a.arg.data) for a in self.infos
+                      if a.kind not in ('gbl_read', 'gbl_red') and a.access is not READ)
+        E = '(op2_i64)%s' % ev
         for a in self.infos:
             p, t = a.pos, a.ctype
+            n = lambda base: '%s%d%s' % (base, p, sfx)
+            # read-only rows of Dats no argument writes go through registers
+            local = (a.access is READ and a.kind in ('direct', 'ind', 'vec') and
+                     id(a.arg.data) not in written and
+                     a.cdim * (a.mdim if a.kind == 'vec' else 1) <= 16 and
+                     not (a.kind == 'direct' and self.mode == 'stream' and self._staged(a)))
             if a.kind == 'gbl_read':
                 call_args.append('g%d' % p)
             elif a.kind == 'gbl_red':
                 if self.mode == 'owner':
-                    decls.append('%s junk%d[%d];' % (t, p, a.cdim))
-                    decls.append('for (int k = 0; k < %d; ++k) junk%d[k] = red%d[k];'
-                                 % (a.cdim, p, p))
-                    call_args.append('(prim ? red%d : junk%d)' % (p, p))
+                    decls.append('%s %s[%d];' % (t, n('junk'), a.cdim))
+                    decls.append('for (int k = 0; k < %d; ++k) %s[k] = red%d[k];'
+                                 % (a.cdim, n('junk'), p))
+                    call_args.append('(prim ? red%d : %s)' % (p, n('junk')))
                 else:
                     call_args.append('red%d' % p)
             elif a.kind == 'direct':
                 if self.mode == 'stream' and self._staged(a):
                     call_args.append('st%d + threadIdx.x * %d' % (p, self._pitch(a)))
                 elif self._row_vector(a):
-                    decls.append('op2_row%d pr%d;' % (p, p))
+                    decls.append('op2_row%d %s;' % (p, n('pr')))
                     if a.access is not WRITE:
-                        compute.append('pr%d = *(const op2_row%d*)(d%d + (op2_i64)e * %d);'
-                                       % (p, p, p, a.cdim))
-                    call_args.append('pr%d.v' % p)
-                    post.append('*(op2_row%d*)(d%d + (op2_i64)e * %d) = pr%d;'
-                                % (p, p, a.cdim, p))
+                        compute.append('%s = *(const op2_row%d*)(d%d + %s * %d);'
+                                       % (n('pr'), p, p, E, a.cdim))
+                    call_args.append('%s.v' % n('pr'))
+                    post.append('*(op2_row%d*)(d%d + %s * %d) = %s;'
+                                % (p, p, E, a.cdim, n('pr')))
+                elif local:
+                    decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
+                    loads.append('for (int k = 0; k < %d; ++k) %s[k] = d%d[%s * %d + k];'
+                                 % (a.cdim, n('rl'), p, E, a.cdim))
+                    call_args.append(n('rl'))
                 else:
-                    call_args.append('d%d + (op2_i64)e * %d' % (p, a.cdim))
+                    call_args.append('d%d + %s * %d' % (p, E, a.cdim))
             elif a.kind == 'ind':
                 if a.race:
-                    decls.append('%s inc%d[%d];' % (t, p, a.cdim))
-                    decls.append('for (int k = 0; k < %d; ++k) inc%d[k] = (%s)0;' % (a.cdim, p, t))
-                    call_args.append('inc%d' % p)
+                    decls.append('%s %s[%d];' % (t, n('inc'), a.cdim))
+                    decls.append('for (int k = 0; k < %d; ++k) %s[k] = (%s)0;' % (a.cdim, n('inc'), t))
+                    call_args.append(n('inc'))
+                elif local:
+                    decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
+                    loads.append('{ const %s* src = d%d + (op2_i64)m%d[%s * %d + %d] * %d; '
+                                 'for (int k = 0; k < %d; ++k) %s[k] = src[k]; }'
+                                 % (t, p, p, E, a.mdim, a.idx, a.cdim, a.cdim, n('rl')))
+                    call_args.append(n('rl'))
                 else:
-                    call_args.append('d%d + (op2_i64)m%d[(op2_i64)e * %d + %d] * %d'
-                                     % (p, p, a.mdim, a.idx, a.cdim))
+                    call_args.append('d%d + (op2_i64)m%d[%s * %d + %d] * %d'
+                                     % (p, p, E, a.mdim, a.idx, a.cdim))
             elif a.kind == 'vec':
-                decls.append('%s* v%d[%d];' % (t, p, a.mdim))
+                decls.append('%s* %s[%d];' % (t, n('v'), a.mdim))
                 if a.race:
-                    decls.append('%s inc%d[%d][%d];' % (t, p, a.mdim, a.cdim))
-                    decls.append('for (int j = 0; j < %d; ++j) { v%d[j] = inc%d[j]; '
-                                 'for (int k = 0; k < %d; ++k) inc%d[j][k] = (%s)0; }'
-                                 % (a.mdim, p, p, a.cdim, p, t))
+                    decls.append('%s %s[%d][%d];' % (t, n('inc'), a.mdim, a.cdim))
+                    decls.append('for (int j = 0; j < %d; ++j) { %s[j] = %s[j]; '
+                                 'for (int k = 0; k < %d; ++k) %s[j][k] = (%s)0; }'
+                                 % (a.mdim, n('v'), n('inc'), a.cdim, n('inc'), t))
+                elif local:
+                    decls.append('%s %s[%d][%d];' % (t, n('rl'), a.mdim, a.cdim))
+                    loads.append('for (int j = 0; j < %d; ++j) { const %s* src = d%d + '
+                                 '(op2_i64)m%d[%s * %d + j] * %d; %s[j] = %s[j]; '
+                                 'for (int k = 0; k < %d; ++k) %s[j][k] = src[k]; }'
+                                 % (a.mdim, t, p, p, E, a.mdim, a.cdim, n('v'), n('rl'),
+                                    a.cdim, n('rl')))
                 else:
-                    compute.append('for (int j = 0; j < %d; ++j) v%d[j] = d%d + '
-                                   '(op2_i64)m%d[(op2_i64)e * %d + j] * %d;'
-                                   % (a.mdim, p, p, p, a.mdim, a.cdim))
-                call_args.append('v%d' % p)
+                    compute.append('for (int j = 0; j < %d; ++j) %s[j] = d%d + '
+                                   '(op2_i64)m%d[%s * %d + j] * %d;'
+                                   % (a.mdim, n('v'), p, p, E, a.mdim, a.cdim))
+                call_args.append(n('v'))
             elif a.kind == 'itspace':
                 ivar = 'i_%d' % a.iidx
                 if a.race:
-                    decls.append('%s inc%d[%d][%d];' % (t, p, a.extent, a.cdim))
+                    decls.append('%s %s[%d][%d];' % (t, n('inc'), a.extent, a.cdim))
                     decls.append('for (int j = 0; j < %d; ++j) for (int k = 0; k < %d; ++k) '
-                                 'inc%d[j][k] = (%s)0;' % (a.extent, a.cdim, p, t))
-                    call_args.append('inc%d[%s]' % (p, ivar))
+                                 '%s[j][k] = (%s)0;' % (a.extent, a.cdim, n('inc'), t))
+                    call_args.append('%s[%s]' % (n('inc'), ivar))
                 else:
-                    call_args.append('d%d + (op2_i64)m%d[(op2_i64)e * %d + %s] * %d'
-                                     % (p, p, a.mdim, ivar, a.cdim))
+                    call_args.append('d%d + (op2_i64)m%d[%s * %d + %s] * %d'
+                                     % (p, p, E, a.mdim, ivar, a.cdim))
             elif a.kind == 'mat':
                 e0 = ext[0] if len(ext) > 0 else 1
                 e1 = ext[1] if len(ext) > 1 else 1
                 if a.vector:
-                    decls.append('%s lm%d[%d][%d][%d][%d];' % (t, p, e0, e1, a.rdim, a.cdim_m))
+                    decls.append('%s %s[%d][%d][%d][%d];' % (t, n('lm'), e0, e1, a.rdim, a.cdim_m))
                     zeros.append('for (int r = 0; r < %d; ++r) for (int c = 0; c < %d; ++c) '
-                                 'lm%d[i_0][i_1][r][c] = (%s)0;' % (a.rdim, a.cdim_m, p, t))
-                    call_args.append('lm%d[i_0][i_1]' % p)
+                                 '%s[i_0][i_1][r][c] = (%s)0;' % (a.rdim, a.cdim_m, n('lm'), t))
+                    call_args.append('%s[i_0][i_1]' % n('lm'))
                 else:
-                    decls.append('%s lm%d[%d][%d];' % (t, p, e0, e1))
-                    zeros.append('lm%d[i_0][i_1] = (%s)0;' % (p, t))
-                    call_args.append('(%s (*)[1])&lm%d[i_0][i_1]' % (t, p))
+                    decls.append('%s %s[%d][%d];' % (t, n('lm'), e0, e1))
+                    zeros.append('%s[i_0][i_1] = (%s)0;' % (n('lm'), t))
+                    call_args.append('(%s (*)[1])&%s[i_0][i_1]' % (t, n('lm')))
         call_args += ['i_%d' % k for k in range(len(ext))]
         opens = ['for (int i_%d = 0; i_%d < %d; ++i_%d) {' % (k, k, e, k)
                  for k, e in enumerate(ext)]
@@ -344,7 +378,9 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         compute += ['}'] * len(opens)
         compute += closing
         compute += post
-        return decls, compute
+        if split:
+            return decls, loads, compute
+        return decls, loads + compute
 
     def _apply_increments(self):
         """Apply private INC increments and Mat entries of element ``e``."""
@@ -438,15 +474,31 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         out = [ind(self._red_init(), 2)]
         for a in staged:
             out.append('  __shared__ %s st%d[%d];' % (a.ctype, a.pos, bs * self._pitch(a)))
+        u = self.stream_unroll()
         # workgroups are dealt to the 8 XCDs round-robin: give each XCD a
         # contiguous run of chunks, so neighbouring chunks share its L2
         out.append('  const int nbk = (int)gridDim.x, bid = (int)blockIdx.x;')
         out.append('  const int xcd = bid & 7, qx = nbk >> 3, rx = nbk & 7;')
         out.append('  const op2_i64 lb = xcd * qx + (xcd < rx ? xcd : rx) + (bid >> 3);')
-        out.append('  const op2_i64 stride = (op2_i64)nbk * %d;' % bs)
-        out.append('  for (op2_i64 base = lb * %d; base < count; base += stride) {' % bs)
+        out.append('  const op2_i64 stride = (op2_i64)nbk * %d;' % (bs * u))
+        out.append('  for (op2_i64 base = lb * %d; base < count; base += stride) {' % (bs * u))
         out.append('    const op2_i64 e = start + base + threadIdx.x;')
         out.append('    const bool active = base + threadIdx.x < count;')
+        if u == 2:
+            # two elements per thread, bs apart: both elements' register
+            # rows are loaded before either kernel runs
+            da, la, ca = self._element_body(split=True)
+            db, lb_, cb = self._element_body(sfx='_b', ev='e_b', split=True)
+            out.append('    const op2_i64 e_b = e + %d;' % bs)
+            out.append('    const bool active_b = base + %d + threadIdx.x < count;' % bs)
+            out.append(ind(da + db, 4))
+            out.append('    if (active_b) {')
+            out.append(ind(la + lb_ + ca + cb, 6))
+            out.append('    } else if (active) {')
+            out.append(ind(la + ca, 6))
+            out.append('    }')
+            out.append('  }')
+            return out
         out.append('    const int nrow = count - base < %d ? (int)(count - base) : %d;' % (bs, bs))
         out.append('    (void)nrow;')
         loads = [a for a in staged if a.access is not WRITE]
@@ -470,6 +522,16 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             out.append('    __syncthreads();')
         out.append('  }')
         return out
+
+    def stream_unroll(self):
+        """Elements per thread and chunk of the direct-loop template: 2 when
+        asked for and the loop stages nothing through LDS and has no
+        iteration space, else 1."""
+        if self.unroll != 2 or self.mode != 'stream' or self.extents:
+            return 1
+        if any(self._staged(a) for a in self.infos):
+            return 1
+        return 2
 
     def owner_columns(self):
         """(arg info, first race column, number of columns) of the race

@@ -800,7 +800,9 @@ class _LaunchEntry(object):
             nparts = plan.nblocks
             grid_stream = 0
         else:
-            grid_stream = max(1, min((n + bs - 1) // bs, 4096))
+            chunk = bs * loop.source.stream_unroll()
+            grid_stream = max(1, min((n + chunk - 1) // chunk,
+                                     int(cfg.get('hip_stream_blocks') or 4096)))
             nparts = grid_stream
         partials = {}
         vals = []
@@ -989,11 +991,12 @@ class ParLoop(base.ParLoop):
         pf = int(cfg.get('hip_owner_prefetch', 1))
         sr = bool(cfg.get('hip_stage_reads', False))
         sw = bool(cfg.get('hip_stage_writes', True))
-        key = key + (mode, bs, pf, sr, sw)
+        un = int(cfg.get('hip_stream_unroll', 1))
+        key = key + (mode, bs, pf, sr, sw, un)
         c = _loop_cache.get(key)
         if c is None:
             src = LoopSource(self.kernel, self._it_space.extents, self.args, consts, mode, bs,
-                             prefetch=pf, stage_reads=sr, stage_writes=sw)
+                             prefetch=pf, stage_reads=sr, stage_writes=sw, unroll=un)
             c = _CompiledLoop(src)
             _loop_cache[key] = c
         return c
