@@ -103,7 +103,9 @@ int op2hip_plan_verify(int nelems, int nblocks, const int* elems, const int* off
  * element naming one of its targets (ascending order) and sums each owned
  * target's increments in that order.  Arrays (which): 0 blk_t0, 1 blk_t1,
  * 2 ex_off, 3 exec (bit 31: not primary), 4 tp, 5 contrib (slot*nac + ac),
- * 6 tg_off, 7 tgt (targets receiving increments, per block).  Scalars
+ * 6 tg_off, 7 tgt (targets receiving increments, per block), 8 rng_lo,
+ * 9 rng_n (a block's primary elements as one run when consecutive: slots
+ * [0, rng_n) are elements rng_lo + slot, exec lists slots rng_n on).  Scalars
  * (field): 0 nactive, 1 max_exec, 2 nac, 3 exec length, 4 contrib length,
  * 5 tgt length.  elem_order (NULL: index order) is the order increments
  * are summed in. */
@@ -120,7 +122,8 @@ int op2hip_owner_plan_destroy(void* plan);
  * into it, in sequential order.  poff[npairs+1] and prog[ncontrib*e1]
  * (uint16 when wide == 0, else int32) are caller-allocated host arrays. */
 int op2hip_mat_program(int nactive, const int* blk_t0, const int* blk_t1, const int* ex_off,
-                       const int* exec, const int* tp, const int* contrib, int nac,
+                       const int* exec, const int* rng_lo, const int* rng_n, const int* tp,
+                       const int* contrib, int nac,
                        const int* cmap, int carity, int e1, int nnodes, const int* rowptr,
                        const int* colidx, int rd, int cd, int wide, int* poff, void* prog);
 

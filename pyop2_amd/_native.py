@@ -88,7 +88,8 @@ _SIGS = {
     'op2hip_owner_plan_scalar': (c_int, [c_void, c_int, P(ctypes.c_int64)]),
     'op2hip_owner_plan_array': (c_int, [c_void, c_int, c_void]),
     'op2hip_owner_plan_destroy': (c_int, [c_void]),
-    'op2hip_mat_program': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void, c_int,
+    'op2hip_mat_program': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void, c_void,
+                                   c_void, c_int,
                                    c_void, c_int, c_int, c_int, c_void, c_void, c_int, c_int,
                                    c_int, c_void, c_void]),
     'op2hip_order_bfs': (c_int, [c_int, c_int, c_void, c_int, c_void]),

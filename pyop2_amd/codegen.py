@@ -124,6 +124,8 @@ class LoopSource(object):
 
     PRELUDE = r'''
 #define OP2_STRIDE(a, idx) a[idx]
+// owner-plan contribution codes: 16-bit when they fit (own_c16), else 32-bit
+#define OP2_CONTRIB(i) (own_c16 ? (int)((const unsigned short*)p_contrib)[i] : ((const int*)p_contrib)[i])
 typedef long long op2_i64;
 template <typename T> __device__ inline T op2_shfl_xor(T v, int m) { return __shfl_xor(v, m, 64); }
 __device__ inline unsigned int op2_shfl_xor(unsigned int v, int m) {
@@ -182,9 +184,11 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             roles += [('plan_elems', None), ('plan_blkmap', None), ('plan_offset', None),
                       ('plan_nelem', None), ('plan_nthrcol', None), ('plan_thrcol', None)]
         elif self.mode == 'owner':
-            roles += [('own_t0', None), ('own_t1', None), ('own_tgoff', None),
+            roles += [('own_t0', None), ('own_t1', None), ('own_rlo', None), ('own_rn', None),
+                      ('own_tgoff', None),
                       ('own_tgt', None), ('own_tgcb', None), ('own_exoff', None), ('own_exec', None),
-                      ('own_tp', None), ('own_contrib', None), ('own_pitch', None), ('own_k', None),
+                      ('own_tp', None), ('own_contrib', None), ('own_c16', None), ('own_pitch', None),
+                      ('own_k', None),
                       ('own_cpitch', None), ('own_poff', None), ('own_prog', None),
                       ('own_wide', None)]
         for a in self.infos:
@@ -581,14 +585,16 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                              % (n, cdim, ac0, cdim, a.pos))
         sub = {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos, 'nac': nac, 'cmax': OWNER_CMAX}
         codes = '''#pragma unroll
-    for (int u = 0; u < %(cmax)d; ++u) w_code[u] = u < w_nc ? p_contrib[w_cb + u] : 0;''' % sub
+    for (int u = 0; u < %(cmax)d; ++u) w_code[u] = u < w_nc ? OP2_CONTRIB(w_cb + u) : 0;''' % sub
         out = ['  extern __shared__ %s op2_lds[];' % t,
                ind(self._red_init(), 2),
                '''  // consecutive owner blocks (neighbouring target ranges) go to one XCD
   const int nb = (int)count, bid = blockIdx.x;
   const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
   const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
-  const int ex0 = p_exoff[b], nex = p_exoff[b + 1] - ex0;
+  // slots [0, rn) run elements rlo + slot (primary); the rest are listed
+  const int rlo = p_rlo[b], rn = p_rn[b];
+  const int ex0 = p_exoff[b], nex = rn + p_exoff[b + 1] - ex0;
   const int g0 = p_tgoff[b], ng = p_tgoff[b + 1] - g0;
   // a block whose every owned target receives increments needs no list
   const int t0 = p_t0[b];
@@ -601,7 +607,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   const int jf = (int)threadIdx.x < ng ? (int)threadIdx.x : ng;
   const int f_t = p_tgt[g0 + jf], f_cb = p_tgcb[g0 + jf], f_ce = p_tgcb[g0 + jf + 1];''')
         out += ['''  for (int l = threadIdx.x; l < nex; l += %d) {
-    const unsigned code = (unsigned)p_exec[ex0 + l];
+    const unsigned code = l < rn ? (unsigned)(rlo + l) : (unsigned)p_exec[ex0 + l - rn];
     const int e = (int)(code & 0x7fffffffu);
     const bool prim = (code >> 31) == 0u;
     (void)prim;''' % bs,
@@ -641,7 +647,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
       }
     }
     for (int c = %(cmax)d; c < w_nc; ++c) {
-      const int cc = p_contrib[w_cb + c];
+      const int cc = OP2_CONTRIB(w_cb + c);
       const int slot = cc / %(nac)d, ac = cc - slot * %(nac)d;
 #pragma unroll
       for (int k = 0; k < %(cdim)d; ++k) acc[k] += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
@@ -684,7 +690,9 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                '''  const int nb = (int)count, bid = blockIdx.x;
   const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
   const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
-  const int ex0 = p_exoff[b], nex = p_exoff[b + 1] - ex0;
+  // slots [0, rn) run elements rlo + slot (primary); the rest are listed
+  const int rlo = p_rlo[b], rn = p_rn[b];
+  const int ex0 = p_exoff[b], nex = rn + p_exoff[b + 1] - ex0;
   const int t0 = p_t0[b], nn = p_t1[b] - t0, kmax = (int)own_k;
   const int pitch = (int)lds_pitch, cpitch = (int)own_cpitch;
   // LDS: per contribution its local-matrix row [%(ent)d][cpitch]; per element
@@ -699,10 +707,10 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   for (int i = threadIdx.x; i <= nn * %(rd)d; i += %(bs)d) s_rp[i] = rp%(p)d[t0 * %(rd)d + i];
   __syncthreads();
   // a contribution code is slot * %(nac)d + row position: it indexes s_inv
-  for (int i = threadIdx.x; i < ncode; i += %(bs)d) s_inv[p_contrib[c0 + i]] = i;
+  for (int i = threadIdx.x; i < ncode; i += %(bs)d) s_inv[OP2_CONTRIB(c0 + i)] = i;
   __syncthreads();
   for (int l = threadIdx.x; l < nex; l += %(bs)d) {
-    const unsigned code = (unsigned)p_exec[ex0 + l];
+    const unsigned code = l < rn ? (unsigned)(rlo + l) : (unsigned)p_exec[ex0 + l - rn];
     const int e = (int)(code & 0x7fffffffu);
     const bool prim = (code >> 31) == 0u;
     (void)prim;''' % sub,
@@ -764,11 +772,12 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                      'const int* __restrict__ p_nthrcol', 'const int* __restrict__ p_thrcol']
         elif self.mode == 'owner':
             decl += ['const int* __restrict__ p_t0', 'const int* __restrict__ p_t1',
+                     'const int* __restrict__ p_rlo', 'const int* __restrict__ p_rn',
                      'const int* __restrict__ p_tgoff', 'const int* __restrict__ p_tgt',
                      'const int* __restrict__ p_tgcb',
                      'const int* __restrict__ p_exoff', 'const int* __restrict__ p_exec',
-                     'const int* __restrict__ p_tp', 'const int* __restrict__ p_contrib',
-                     'op2_i64 lds_pitch', 'op2_i64 own_k', 'op2_i64 own_cpitch',
+                     'const int* __restrict__ p_tp', 'const void* __restrict__ p_contrib',
+                     'op2_i64 own_c16', 'op2_i64 lds_pitch', 'op2_i64 own_k', 'op2_i64 own_cpitch',
                      'const int* __restrict__ p_poff', 'const void* __restrict__ p_prog',
                      'op2_i64 own_wide']
         for a in self.infos:

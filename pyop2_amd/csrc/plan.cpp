@@ -413,7 +413,7 @@ namespace {
 struct OwnerPlan {
   int nelems = 0, ntargets = 0, nac = 0, k = 0;
   int nactive = 0, max_exec = 0;
-  std::vector<int> blk_t0, blk_t1, ex_off, exec, tp, contrib, tg_off, tgt;
+  std::vector<int> blk_t0, blk_t1, ex_off, exec, tp, contrib, tg_off, tgt, rng_lo, rng_n;
 };
 
 }  // namespace
@@ -435,7 +435,12 @@ struct OwnerPlan {
 // Race columns ("ac"): race map i contributes columns idxs[i] (or all of its
 // dims[i] columns when idxs[i] < 0), numbered in order across maps.
 //   blk_t0/blk_t1[nactive]  owned target range of active block b
-//   ex_off[nactive+1], exec  executed elements; bit 31 set when not primary
+//   rng_lo/rng_n[nactive]    when block b's primary elements are one run of
+//                            consecutive elements, that run: slots
+//                            [0, rng_n) are elements rng_lo + slot (else
+//                            rng_n = 0)
+//   ex_off[nactive+1], exec  the block's other executed elements, slots
+//                            rng_n onward; bit 31 set when not primary
 //   tp[ntargets+1], contrib  per target, its (slot * nac + ac) increments in
 //                            ascending element order (slot: block-local)
 //   tg_off[nactive+1], tgt   the targets of each block that receive increments
@@ -492,7 +497,7 @@ int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* cons
   p->contrib.reserve(inc.size());
   p->ex_off.push_back(0);
   p->tg_off.push_back(0);
-  std::vector<int> els;
+  std::vector<int> els, prim, nonp;
   for (int t0 = 0; t0 < ntargets; t0 += k) {
     int t1 = std::min(ntargets, t0 + k);
     els.clear();
@@ -500,20 +505,37 @@ int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* cons
       for (int q = cnt[t]; q < cnt[t + 1]; ++q) els.push_back(static_cast<int>(inc[q] / nac));
     std::sort(els.begin(), els.end());
     els.erase(std::unique(els.begin(), els.end()), els.end());
+    // primary elements (first race column owned here) as one run when they
+    // are consecutive, then the others; slots number them in that order
+    prim.clear();
+    nonp.clear();
+    for (int e : els) (target(e, 0) / k == t0 / k ? prim : nonp).push_back(e);
+    const bool run = !prim.empty() && prim.back() - prim.front() + 1 == static_cast<int>(prim.size());
+    const int rn = run ? static_cast<int>(prim.size()) : 0;
+    auto slot_of = [&](int e) -> int {
+      if (!run) return static_cast<int>(std::lower_bound(els.begin(), els.end(), e) - els.begin());
+      if (e >= prim.front() && e <= prim.back()) return e - prim.front();
+      return rn + static_cast<int>(std::lower_bound(nonp.begin(), nonp.end(), e) - nonp.begin());
+    };
     for (int t = t0; t < t1; ++t) {
       for (int q = cnt[t]; q < cnt[t + 1]; ++q) {
         int e = static_cast<int>(inc[q] / nac);
         int ac = static_cast<int>(inc[q] % nac);
-        int slot = static_cast<int>(std::lower_bound(els.begin(), els.end(), e) - els.begin());
-        p->contrib.push_back(slot * nac + ac);
+        p->contrib.push_back(slot_of(e) * nac + ac);
       }
       p->tp[t + 1] = static_cast<int>(p->contrib.size());
     }
     if (els.empty()) continue;
-    for (int e : els) {
-      bool primary = target(e, 0) / k == t0 / k;
-      p->exec.push_back(primary ? e : static_cast<int>(static_cast<unsigned>(e) | 0x80000000u));
+    if (run) {
+      for (int e : nonp) p->exec.push_back(static_cast<int>(static_cast<unsigned>(e) | 0x80000000u));
+    } else {
+      for (int e : els) {
+        bool primary = target(e, 0) / k == t0 / k;
+        p->exec.push_back(primary ? e : static_cast<int>(static_cast<unsigned>(e) | 0x80000000u));
+      }
     }
+    p->rng_lo.push_back(run ? prim.front() : 0);
+    p->rng_n.push_back(rn);
     p->blk_t0.push_back(t0);
     p->blk_t1.push_back(t1);
     p->ex_off.push_back(static_cast<int>(p->exec.size()));
@@ -543,7 +565,8 @@ int op2hip_owner_plan_scalar(void* plan, int field, int64_t* out) {
   return OP2HIP_OK;
 }
 
-// 0 blk_t0, 1 blk_t1, 2 ex_off, 3 exec, 4 tp, 5 contrib, 6 tg_off, 7 tgt
+// 0 blk_t0, 1 blk_t1, 2 ex_off, 3 exec, 4 tp, 5 contrib, 6 tg_off, 7 tgt,
+// 8 rng_lo, 9 rng_n
 int op2hip_owner_plan_array(void* plan, int which, int* dst) {
   OwnerPlan* p = static_cast<OwnerPlan*>(plan);
   OP2HIP_REQUIRE(p && dst, "null plan");
@@ -557,6 +580,8 @@ int op2hip_owner_plan_array(void* plan, int which, int* dst) {
     case 5: v = &p->contrib; break;
     case 6: v = &p->tg_off; break;
     case 7: v = &p->tgt; break;
+    case 8: v = &p->rng_lo; break;
+    case 9: v = &p->rng_n; break;
     default: OP2HIP_REQUIRE(false, "unknown owner plan array %d", which);
   }
   if (!v->empty()) memcpy(dst, v->data(), v->size() * sizeof(int));
@@ -936,7 +961,8 @@ extern "C" {
 // c * e1 + j as uint16 (wide == 0) or int32 (wide == 1).  Fails if an
 // element names a column the pattern lacks or a code does not fit.
 int op2hip_mat_program(int nactive, const int* blk_t0, const int* blk_t1, const int* ex_off,
-                       const int* exec, const int* tp, const int* contrib, int nac,
+                       const int* exec, const int* rng_lo, const int* rng_n, const int* tp,
+                       const int* contrib, int nac,
                        const int* cmap, int carity, int e1, int nnodes, const int* rowptr,
                        const int* colidx, int rd, int cd, int wide, int* poff, void* prog) {
   OP2HIP_REQUIRE(nactive >= 0 && nac > 0 && carity > 0 && e1 > 0 && e1 <= carity && rd > 0 &&
@@ -966,7 +992,8 @@ int op2hip_mat_program(int nactive, const int* blk_t0, const int* blk_t1, const 
       for (int n = blk_t0[b]; n < blk_t1[b]; ++n)
         for (int c = tp[n]; c < tp[n + 1]; ++c) {
           const int slot = contrib[c] / nac;
-          const int e = static_cast<int>(static_cast<unsigned>(exec[ex_off[b] + slot]) & 0x7fffffffu);
+          const int e = slot < rng_n[b] ? rng_lo[b] + slot
+              : static_cast<int>(static_cast<unsigned>(exec[ex_off[b] + slot - rng_n[b]]) & 0x7fffffffu);
           for (int j = 0; j < e1; ++j) {
             const long long p = pair_of(n, cmap[static_cast<long long>(e) * carity + j]);
             if (p < 0) { bad = 1; continue; }
@@ -988,7 +1015,8 @@ int op2hip_mat_program(int nactive, const int* blk_t0, const int* blk_t1, const 
       for (int n = blk_t0[b]; n < blk_t1[b]; ++n)
         for (int c = tp[n]; c < tp[n + 1]; ++c) {
           const int slot = contrib[c] / nac;
-          const int e = static_cast<int>(static_cast<unsigned>(exec[ex_off[b] + slot]) & 0x7fffffffu);
+          const int e = slot < rng_n[b] ? rng_lo[b] + slot
+              : static_cast<int>(static_cast<unsigned>(exec[ex_off[b] + slot - rng_n[b]]) & 0x7fffffffu);
           const long long code0 = static_cast<long long>(c - cbase) * e1;
           for (int j = 0; j < e1; ++j) {
             const long long p = pair_of(n, cmap[static_cast<long long>(e) * carity + j]);

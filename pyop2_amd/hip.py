@@ -482,6 +482,15 @@ class Plan(object):
         self.colour_start = np.concatenate([[0], np.cumsum(self.ncolblk)]).astype(np.int64)
         self._dev = None
 
+    def block_elements(self, b):
+        """(elements in slot order, primary flags) of active block ``b``."""
+        ex = self.exec[self.ex_off[b]:self.ex_off[b + 1]].view(np.uint32)
+        rn = int(self.rng_n[b])
+        els = np.concatenate([np.arange(self.rng_lo[b], self.rng_lo[b] + rn),
+                              (ex & 0x7fffffff).astype(np.int64)])
+        prim = np.concatenate([np.ones(rn, dtype=bool), (ex >> 31) == 0])
+        return els, prim
+
     def device_arrays(self):
         """The plan arrays the coloured loops read, uploaded on first use."""
         if self._dev is None:
@@ -554,7 +563,9 @@ class OwnerPlan(object):
     elements naming its targets and sums each owned target's increments in
     ascending element order."""
 
-    ARRAYS = ('blk_t0', 'blk_t1', 'ex_off', 'exec', 'tp', 'contrib', 'tg_off', 'tgt', 'tg_cb')
+    NATIVE = ('blk_t0', 'blk_t1', 'ex_off', 'exec', 'tp', 'contrib', 'tg_off', 'tgt',
+              'rng_lo', 'rng_n')
+    ARRAYS = NATIVE + ('tg_cb',)
 
     def __init__(self, n, ntargets, cols, k, elem_order=None):
         self.n, self.ntargets, self.k = n, ntargets, k
@@ -575,8 +586,9 @@ class OwnerPlan(object):
             self.nactive, self.max_exec, self.nac, nexec, ncontrib, ntgt = sc
             sizes = {'blk_t0': self.nactive, 'blk_t1': self.nactive,
                      'ex_off': self.nactive + 1, 'exec': nexec, 'tp': ntargets + 1,
-                     'contrib': ncontrib, 'tg_off': self.nactive + 1, 'tgt': ntgt}
-            for w, name in enumerate(self.ARRAYS[:8]):
+                     'contrib': ncontrib, 'tg_off': self.nactive + 1, 'tgt': ntgt,
+                     'rng_lo': self.nactive, 'rng_n': self.nactive}
+            for w, name in enumerate(self.NATIVE):
                 arr = np.zeros(sizes[name], dtype=np.int32)
                 if arr.size:
                     nat.check(nat.lib.op2hip_owner_plan_array(
@@ -587,6 +599,9 @@ class OwnerPlan(object):
         # contribution offsets by touched-target position (targets without
         # increments have empty ranges, so the next entry ends each range)
         self.tg_cb = np.append(self.tp[self.tgt], self.tp[-1]).astype(np.int32)
+        # elements run per block (run of primaries plus listed ones)
+        self.nexec = np.diff(self.ex_off) + self.rng_n if self.nactive else \
+            np.zeros(0, dtype=np.int64)
         # contribution codes per block (its owned targets' range of contrib)
         self.max_codes = int((self.tp[self.blk_t1] - self.tp[self.blk_t0]).max()) \
             if self.nactive else 0
@@ -595,11 +610,27 @@ class OwnerPlan(object):
         self._pad = {'tgt': 0, 'tg_cb': int(self.tp[-1])}
         self._dev = None
 
+    def block_elements(self, b):
+        """(elements in slot order, primary flags) of active block ``b``."""
+        ex = self.exec[self.ex_off[b]:self.ex_off[b + 1]].view(np.uint32)
+        rn = int(self.rng_n[b])
+        els = np.concatenate([np.arange(self.rng_lo[b], self.rng_lo[b] + rn),
+                              (ex & 0x7fffffff).astype(np.int64)])
+        prim = np.concatenate([np.ones(rn, dtype=bool), (ex >> 31) == 0])
+        return els, prim
+
+    @property
+    def c16(self):
+        """Contribution codes fit 16 bits (the device copy is uint16)."""
+        return self.max_exec * self.nac <= 0xffff
+
     def device_arrays(self):
         if self._dev is None:
             self._dev = {}
             for name in self.ARRAYS:
                 a = getattr(self, name)
+                if name == 'contrib' and self.c16:
+                    a = a.astype(np.uint16)
                 if name in self._pad:
                     a = np.append(a, np.int32(self._pad[name])).astype(np.int32)
                 d = _dev_bytes(a.nbytes)
@@ -626,7 +657,8 @@ def mat_program_host(plan, cmap, e1, nnodes, rowptr, colidx, rd, cd):
     poff = np.empty(npairs + 1, dtype=np.int32)
     prog = np.empty(len(plan.contrib) * e1, dtype=np.int32 if wide else np.uint16)
     arrs = [np.ascontiguousarray(x, dtype=np.int32) for x in
-            (plan.blk_t0, plan.blk_t1, plan.ex_off, plan.exec, plan.tp, plan.contrib)]
+            (plan.blk_t0, plan.blk_t1, plan.ex_off, plan.exec, plan.rng_lo, plan.rng_n, plan.tp,
+             plan.contrib)]
     P = lambda x: x.ctypes.data_as(ctypes.c_void_p)
     nat.check(nat.lib.op2hip_mat_program(
         plan.nactive, *[P(x) for x in arrs], plan.nac, P(cmap), cmap.shape[1], e1, nnodes,
@@ -699,7 +731,7 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None)
         p = build(kk)
         if p.nactive:
             # elements run and LDS bytes per owned target, from the trial
-            per_target = (p.ex_off[-1] / float(p.nactive)) / kk
+            per_target = (p.nexec.sum() / float(p.nactive)) / kk
             lds_per_target = lds_of(p) / float(kk)
             fill = cfg.get('hip_owner_fill') or 0.9
             kk = min(1 << 16, max(1, int(fill * block / max(per_target, 1e-9))))
@@ -711,7 +743,7 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None)
     if lds_of(p) > limit:
         _owner_cache[key] = None
         return None
-    if p.nactive and p.ex_off[-1] / float(p.nactive) < 32 and not k_cfg:
+    if p.nactive and p.nexec.sum() / float(p.nactive) < 32 and not k_cfg:
         # blocks would run too few elements to keep a workgroup busy
         _owner_cache[key] = None
         return None
@@ -775,6 +807,9 @@ class _LaunchEntry(object):
         for a in args:
             if a._is_direct and a.data.dataset is not iterset:
                 layout.link(a.data.dataset, iterset)
+        if mode == 'owner':
+            m0, i0 = _owner_columns(args)[0]
+            layout.prefer(iterset, m0, max(i0, 0))
         layout.of(iterset)
         self.to_device = []
         for a in args:
@@ -848,13 +883,14 @@ class _LaunchEntry(object):
                     ma = [x for x in args if x._is_mat][0]
                     prog = _mat_program(plan, ma, pl._it_space.extents[1])
                 names = {'own_t0': 'blk_t0', 'own_t1': 'blk_t1', 'own_tgcb': 'tg_cb',
+                         'own_rlo': 'rng_lo', 'own_rn': 'rng_n',
                          'own_tgoff': 'tg_off', 'own_tgt': 'tgt', 'own_exoff': 'ex_off',
                          'own_exec': 'exec', 'own_tp': 'tp', 'own_contrib': 'contrib'}
                 argv = [0 if r == 'start' else plan.nactive if r == 'count' else
                         plan.max_exec if r == 'own_pitch' else plan.k if r == 'own_k' else
                         max(plan.max_codes, 1) if r == 'own_cpitch' else
                         _ptr(prog[0]) if r == 'own_poff' else _ptr(prog[1]) if r == 'own_prog' else
-                        prog[2] if r == 'own_wide' else
+                        prog[2] if r == 'own_wide' else int(plan.c16) if r == 'own_c16' else
                         _ptr(pdev[names[r]]) if isinstance(r, str) else r for r in vals]
                 if plan.nactive:
                     launches.append((loop.fn, plan.nactive, bs, _argv(argv), lds))

@@ -6,6 +6,8 @@ different cache line, and an MI355X loop moves many times its unique bytes.
 The hip backend therefore stores each Set's device data in a *layout order*
 chosen for locality, decided once per Set at its first device use:
 
+0. an iteration Set of an owner-computes loop is sorted by the position of
+   the target its race map names first (edges follow their first cell);
 1. a Set that maps into an already laid-out Set is sorted by the first
    position its elements name there (edges follow their cells);
 2. a Set that an already laid-out Set maps into takes the order in which
@@ -121,6 +123,14 @@ def decided(s):
     return _group(s).layout is not None
 
 
+def prefer(s, m, col=0):
+    """Ask that Set ``s``, if still undecided, be sorted by the position
+    of the target its map ``m`` names in column ``col`` (an owner-computes
+    loop over ``s`` then finds each block's primary elements consecutive)."""
+    if not decided(s):
+        s._layout_pref = (m, col)
+
+
 def of(s):
     """The layout of Set ``s``, deciding it (for its whole group) if needed."""
     g = _group(s)
@@ -140,6 +150,16 @@ def _decide(g):
     maps_from = [m for s in sets for m in getattr(s, '_maps_from', [])
                  if len(m.values) and m.iterset.total_size == n]
     maps_to = [m for s in sets for m in getattr(s, '_maps_to', []) if len(m.values)]
+    # 0. a preferred map (set by an owner-computes loop): sort by the
+    #    position of the target in the preferred column
+    for s in sets:
+        pref = getattr(s, '_layout_pref', None)
+        if pref is not None and len(pref[0].values) and pref[0].iterset.total_size == n:
+            m, col = pref
+            T = of(m.dataset)
+            if _group(m.dataset) is not g and not T.identity:
+                key = T.iperm[np.asarray(m.values)[:, col]]
+                return Layout(np.argsort(key, kind='stable'))
     # 1. follow the first laid-out target set
     for m in maps_from:
         T = m.dataset

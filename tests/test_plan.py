@@ -117,10 +117,14 @@ def check_owner(p, cols, n, ntarget):
     for b in range(p.nactive):
         t0, t1 = p.blk_t0[b], p.blk_t1[b]
         owned[t0:t1] += 1
-        ex = p.exec[p.ex_off[b]:p.ex_off[b + 1]].view(np.uint32)
-        els = (ex & 0x7fffffff).astype(np.int64)
-        assert (np.diff(els) > 0).all()
-        primary[els[(ex >> 31) == 0]] += 1
+        els, prim = p.block_elements(b)
+        assert len(np.unique(els)) == len(els)
+        rn = int(p.rng_n[b])
+        # a run of consecutive primaries, then the listed elements ascending
+        assert (np.diff(els[:rn]) == 1).all() and prim[:rn].all()
+        assert (np.diff(els[rn:]) > 0).all()
+        assert len(els) <= p.max_exec
+        primary[els[prim]] += 1
         for t in range(t0, t1):
             prev = -1
             for c in p.contrib[p.tp[t]:p.tp[t + 1]]:
@@ -154,6 +158,20 @@ def test_owner_airfoil_edges():
     cols = [(m.pecell, 0), (m.pecell, 1)]
     p = OwnerPlan(m.nedge, m.ncell, [(v, 2, i) for v, i in cols], 64)
     check_owner(p, cols, m.nedge, m.ncell)
+
+
+def test_owner_primary_runs():
+    """Edges sorted by their first cell: every block's primary edges are one
+    run (no list entries for them), and the plan stays valid."""
+    m = meshes.QuadMesh(60, 30, seed=5)
+    order = np.argsort(m.pecell[:, 0], kind='stable')
+    pec = np.ascontiguousarray(m.pecell[order])
+    cols = [(pec, 0), (pec, 1)]
+    p = OwnerPlan(m.nedge, m.ncell, [(v, 2, i) for v, i in cols], 64)
+    check_owner(p, cols, m.nedge, m.ncell)
+    assert (p.rng_n > 0).all()
+    assert p.rng_n.sum() == m.nedge
+    assert ((p.exec.view(np.uint32) >> 31) == 1).all()
 
 
 def _node_csr(elem_node, nnodes, rd, cd):
@@ -203,7 +221,7 @@ def test_mat_scatter_program(rd, cd, k):
                 for code in prog[poff[pp]:poff[pp + 1]]:
                     c, j = divmod(int(code), 3)
                     slot, i0 = divmod(int(p.contrib[cbase + c]), p.nac)
-                    e = int(p.exec[p.ex_off[b] + slot]) & 0x7fffffff
+                    e = int(p.block_elements(b)[0][slot])
                     assert en[e, i0] == node and en[e, j] == m
                     got.append((rank[e], j))
                     seen.add((e, i0, j))
