@@ -97,13 +97,19 @@ class CGBatch(object):
         self.load(systems, stream)
         maxit = max(s.maxit for s in systems) if systems else 0
         done_iters = 0
+        ndone = 0
         while True:
             k = chunk if done_iters else min(chunk, 4)
             self.step(k, stream)
             done_iters += k
             out = self.read(stream)
-            if all(s.state == SYS_DONE for s in out) or done_iters > maxit + chunk:
+            d = sum(s.state == SYS_DONE for s in out)
+            if d == len(out) or done_iters > maxit + chunk:
                 return out
+            if d > ndone:
+                # drop finished systems' rows from the launches
+                self.load(out, stream)
+                ndone = d
 
     def __del__(self):
         try:

@@ -357,7 +357,7 @@ int op2hip_cgb_destroy(void* h) {
 
 // Load the batch's systems (host array of nsys descriptors; state NEW for
 // systems to start, RUNNING/DONE to keep, EMPTY for unused slots) and
-// rebuild the chunk list.  Solver scalars of RUNNING/DONE systems are taken
+// rebuild the chunk list (NEW and RUNNING systems only).  Solver scalars of RUNNING/DONE systems are taken
 // from the descriptors, so a caller that keeps systems across calls passes
 // back what op2hip_cgb_read returned.
 int op2hip_cgb_load(void* h, int nsys, const op2hip_cg_system* systems, void* stream) {
@@ -369,7 +369,8 @@ int op2hip_cgb_load(void* h, int nsys, const op2hip_cg_system* systems, void* st
   for (int k = 0; k < nsys; ++k) {
     Sys& sy = b->h_sys[k];
     sy.chunk0 = static_cast<int>(chunks.size());
-    sy.nchunks = sy.state == SYS_EMPTY ? 0 : (sy.n + kChunk - 1) / kChunk;
+    // finished and empty slots get no chunks: launches cover live rows only
+    sy.nchunks = (sy.state == SYS_EMPTY || sy.state == SYS_DONE) ? 0 : (sy.n + kChunk - 1) / kChunk;
     for (int c = 0; c < sy.nchunks; ++c) chunks.push_back({k, c});
   }
   OP2HIP_REQUIRE(static_cast<long long>(chunks.size()) <= b->max_chunks,

@@ -25,40 +25,51 @@ def tet_mesh_of(nelem, seed):
     return meshes.TetMesh(n, seed=seed)
 
 
-def batch_suite(op2, nmesh=64, emin=2000, emax=200000, rtol=1e-10, seed=0, rounds=2):
+def batch_suite(op2, nmesh=64, emin=2000, emax=200000, rtol=1e-10, seed=0, rounds=2,
+                progress=None):
+    """``progress(msg)``, when given, is called as each phase (and every 32
+    meshes of the one-at-a-time path) completes."""
     import torch
     from ..batch import MeshBatch
+    say = progress or (lambda msg: None)
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(emin), np.log(emax), nmesh))
     ms = [tri_mesh_of(s, int(rng.integers(1 << 30))) for s in sizes]
     fs = [rng.random(m.nvert) for m in ms]
     ks = [rng.lognormal(0.0, 1.0, m.nvert) for m in ms]
     t = time.perf_counter()
+    say('meshes generated: %d elements' % sum(m.nelem for m in ms))
     mix = MeshBatch(op2, ms, fs, ks, name='bsuite')
     mix.assemble()
     mix.solve(rtol=rtol)
     torch.cuda.synchronize()
     setup_b = time.perf_counter() - t
+    say('batched warm-up done (%.1f s)' % setup_b)
     t = time.perf_counter()
     for _ in range(rounds):
         mix.assemble()
         out = mix.solve(rtol=rtol)
     torch.cuda.synchronize()
     tb = (time.perf_counter() - t) / rounds
+    say('batched timed: %.3f s per round' % tb)
     assert all(r > 0 for _, _, r in out)
     singles = [MeshBatch(op2, [m], [f], [k], name='bs%d' % i)
                for i, (m, f, k) in enumerate(zip(ms, fs, ks))]
     t = time.perf_counter()
-    for s1 in singles:
+    for i, s1 in enumerate(singles):
         s1.assemble()
         s1.solve(rtol=rtol, chunk=16)
+        if (i + 1) % 32 == 0:
+            say('one-at-a-time warm-up: %d meshes' % (i + 1))
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t
     t = time.perf_counter()
     for _ in range(rounds):
-        for s1 in singles:
+        for i, s1 in enumerate(singles):
             s1.assemble()
             s1.solve(rtol=rtol, chunk=16)
+            if (i + 1) % 32 == 0:
+                say('one-at-a-time timed: %d meshes' % (i + 1))
     torch.cuda.synchronize()
     ts = (time.perf_counter() - t) / rounds
     same = all(np.array_equal(s1.solution(0), mix.solution(k)) for k, s1 in enumerate(singles))
