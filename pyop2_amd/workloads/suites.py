@@ -96,25 +96,31 @@ def make_pool(op2, n2d=8, n3d=8, tri_range=(8000, 500000), tet_range=(50000, 200
 
 
 def serve_suite(op2, pool, rates=(100, 1000, 5000), nreq=64, budget_gb=64.0, max_active=64,
-                iters=16, rtol=1e-10, seed=0):
+                iters=16, rtol=1e-10, seed=0, progress=None):
+    """``progress(msg)``, when given, is called as each phase completes."""
     import torch
     from .. import serve
+    say = progress or (lambda msg: None)
     out = {'pool': len(pool), 'pool_elements': int(sum(m.mesh.nelem for m in pool)),
            'rtol': rtol, 'budget_GB': budget_gb, 'rates': {}}
     # warm every mesh (compile, plans, slots)
     warm = [serve.Request(k, 0.0, k, 12345 + k) for k in range(len(pool))]
     serve.Server(pool, budget_gb * 1e9, max_active, iters, rtol).run(warm)
+    say('served warm-up done')
     serve.serve_sequential(pool, [serve.Request(k, 0.0, k, 1 + k) for k in range(len(pool))],
                            rtol=rtol)
     torch.cuda.synchronize()
+    say('sequential warm-up done')
     for lam in rates:
         reqs = serve.poisson_requests(nreq, lam, len(pool), seed=seed + lam)
         server = serve.Server(pool, budget_gb * 1e9, max_active, iters, rtol)
         server.run(reqs)
         a = serve.latency_stats(reqs)
+        say('rate %g: served %s' % (lam, a))
         base = serve.poisson_requests(nreq, lam, len(pool), seed=seed + lam)
         serve.serve_sequential(pool, base, rtol=rtol)
         b = serve.latency_stats(base)
+        say('rate %g: sequential %s' % (lam, b))
         out['rates'][str(lam)] = {
             'served': {k: round(v, 3) for k, v in a.items()},
             'sequential': {k: round(v, 3) for k, v in b.items()},
