@@ -41,6 +41,8 @@ def parse():
                    help='approximate CPU time budget of the oracle baseline (0: skip)')
     p.add_argument('--partition-size', type=int, default=0)
     p.add_argument('--block-size', type=int, default=0)
+    p.add_argument('--autotune', type=int, default=0,
+                   help='tune launch parameters per loop and mesh-size bucket (1: on)')
     p.add_argument('--owner-block-size', type=int, default=0)
     p.add_argument('--stream-unroll', type=int, default=0,
                    help='elements per thread of direct loops (0: the default)')
@@ -101,6 +103,8 @@ def main():
         opts['hip_stage_writes'] = bool(args.stage_writes)
     if args.layout_order:
         opts['hip_layout_order'] = args.layout_order
+    if args.autotune:
+        opts['hip_autotune'] = True
     if args.stream_unroll:
         opts['hip_stream_unroll'] = args.stream_unroll
     if args.stream_blocks:
@@ -185,6 +189,7 @@ def main():
                    'unique_bytes_per_step': int(step_bytes),
                    'pct_of_hbm_peak': round(100.0 * value / HBM_PEAK_GBS, 2),
                    'graph': bool(args.graph),
+                   'autotune': _tuned() if args.autotune else False,
                    'partition_size': hip.cfg.get('hip_partition_size') or 'auto'},
         'roofline': {'bound': 'hbm', 'kernel': 'op2_res_calc_owner (one res_calc par_loop)',
                      'achieved': round(res_gbs, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
@@ -212,6 +217,13 @@ def main():
     if extra:
         out['extra'] = extra
     print(json.dumps(out))
+
+
+def _tuned():
+    """Parameters the autotuner chose, per loop signature."""
+    from pyop2_amd import autotune
+    return {k.split('/')[0] + ' ' + k.split('/')[-1]: v['params']
+            for k, v in autotune.table().items()}
 
 
 def _profiled_traffic():

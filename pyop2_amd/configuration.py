@@ -23,7 +23,7 @@ DEFAULTS = {
     'hip_partition_size': 0,      # 0: taken from the autotuner table
     'hip_block_size': 0,          # 0: taken from the autotuner table
     'hip_reorder': True,          # locality-ordered partitions for indirect loops
-    'hip_autotune': False,
+    'hip_autotune': False,        # tune launch parameters per loop and mesh-size bucket
     'hip_stream_blocks': 4096,
     'hip_stream_unroll': 1,       # elements per thread and chunk of direct loops (1 or 2)    # most workgroups of a direct loop (more rows: grid-stride)
     'hip_owner_fill': 0.9,        # owner blocks run about this many elements per thread
@@ -35,6 +35,9 @@ DEFAULTS = {
     'hip_layout_order': 'bisect',  # adjacency order of sets no map orders: 'bisect' or 'bfs'
     'hip_cache_dir': None,        # on-disk code-object cache (None: ~/.cache/pyop2_amd)
 }
+
+
+_MISSING = object()
 
 
 class ConfigModule(types.ModuleType):
@@ -65,6 +68,26 @@ class ConfigModule(types.ModuleType):
     def reset(self):
         """Forget all configuration entries."""
         self._config = None
+
+    def override(self, **kwargs):
+        """Context manager setting entries for the duration of a block."""
+        mod = self
+
+        class _Override(object):
+            def __enter__(self):
+                if mod._config is None:
+                    mod._config = dict(DEFAULTS)
+                self.saved = {k: mod._config.get(k, _MISSING) for k in kwargs}
+                mod._config.update(kwargs)
+
+            def __exit__(self, *exc):
+                for k, v in self.saved.items():
+                    if v is _MISSING:
+                        mod._config.pop(k, None)
+                    else:
+                        mod._config[k] = v
+                return False
+        return _Override()
 
     def get(self, key, default=None):
         """Return an option, falling back to the defaults before configure()."""

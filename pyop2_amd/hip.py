@@ -33,6 +33,7 @@ from . import _native as nat
 from . import jit
 from . import cg
 from . import layout
+from . import autotune
 from .base import (READ, WRITE, RW, INC, MIN, MAX, IdentityMap, IterationIndex, i,
                    IterationSpace, Halo, Access)
 from .codegen import LoopSource
@@ -710,7 +711,8 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None)
     cols = _owner_columns(args)
     target = cols[0][0].dataset
     k_cfg = k or cfg.get('hip_owner_targets')
-    key = (id(iterset), n, k_cfg, block, tuple((id(m), i) for m, i in cols))
+    key = (id(iterset), n, k_cfg, block, cfg.get('hip_owner_fill'),
+           tuple((id(m), i) for m, i in cols))
     if key in _owner_cache:
         return _owner_cache[key]
     lay = layout.of(iterset)
@@ -1037,25 +1039,93 @@ class ParLoop(base.ParLoop):
             _loop_cache[key] = c
         return c
 
-    def compute(self):
+    def _entry(self, mode):
+        """A launch entry for ``mode`` under the current configuration
+        (owner loops whose plan does not fit fall back to colouring)."""
         bs = _block_size()
         obs = cfg.get('hip_owner_block_size') or bs
-        key = (id(self.kernel), self._it_space.extents, id(self._it_space.iterset), bs, obs,
-               cfg.get('hip_partition_size'), cfg.get('hip_reorder', True),
-               tuple(_arg_key(a) for a in self.args),
-               tuple((c.name, id(c)) for c in base.Const._definitions()))
-        ent = _call_cache.get(key)
-        if ent is None:
-            if len(_call_cache) >= _CALL_CACHE_MAX:
-                _call_cache.pop(next(iter(_call_cache)))
+        try:
+            return _LaunchEntry(self, mode, obs if mode == 'owner' else bs)
+        except _NoOwnerPlan:
+            return _LaunchEntry(self, 'coloured', bs)
+
+    def compute(self):
+        params = {}
+        if cfg.get('hip_autotune'):
             mode = self._mode()
-            try:
-                ent = _LaunchEntry(self, mode, obs if mode == 'owner' else bs)
-            except _NoOwnerPlan:
-                ent = _LaunchEntry(self, 'coloured', bs)
-            _call_cache[key] = ent
+            params = autotune.params_for(self, mode, lambda cand: _trial(self, mode, cand))
+        with cfg.override(**params):
+            key = (id(self.kernel), self._it_space.extents, id(self._it_space.iterset),
+                   tuple(cfg.get(k) for k in _VARIANT_KEYS),
+                   tuple(_arg_key(a) for a in self.args),
+                   tuple((c.name, id(c)) for c in base.Const._definitions()))
+            ent = _call_cache.get(key)
+            if ent is None:
+                if len(_call_cache) >= _CALL_CACHE_MAX:
+                    _call_cache.pop(next(iter(_call_cache)))
+                ent = self._entry(self._mode())
+                _call_cache[key] = ent
         ent.run()
         self.maybe_set_halo_update_needed()
+
+
+# configuration entries that shape a loop's launches (part of the call key)
+_VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill',
+                 'hip_owner_targets', 'hip_owner', 'hip_partition_size', 'hip_reorder',
+                 'hip_stage_reads', 'hip_stage_writes', 'hip_stream_unroll',
+                 'hip_stream_blocks', 'hip_owner_prefetch')
+
+
+def _written_buffers(pl):
+    """Device tensors a loop writes (Dats, reduced Globals, Mat values)."""
+    out = []
+    for a in pl.args:
+        if a._is_mat:
+            out.append((a.data, a.data._dev_vals))
+        elif (a._is_dat and a.access is not READ) or a._is_global_reduction:
+            a.data._to_device()
+            out.append((a.data, a.data._dev))
+    return out
+
+
+def _trial(pl, mode, params, budget_s=2e-3):
+    """Seconds per call of ``pl`` launched under ``params``; the data it
+    writes is restored afterwards."""
+    with cfg.override(**params):
+        ent = pl._entry(mode)
+    bufs = _written_buffers(pl)
+    saved = [(d, t.clone(), getattr(d, '_state', None)) for d, t in bufs]
+    s = ctypes.c_void_p(stream())
+    ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for e in ev:
+        nat.check(nat.lib.op2hip_event_create(ctypes.byref(e)), 'event')
+    try:
+        ent.run()
+        torch.cuda.synchronize()
+
+        def timed(reps):
+            nat.lib.op2hip_event_record(ev[0], s)
+            for _ in range(reps):
+                ent.run()
+            nat.lib.op2hip_event_record(ev[1], s)
+            torch.cuda.synchronize()
+            ms = ctypes.c_float()
+            nat.check(nat.lib.op2hip_event_elapsed_ms(ev[0], ev[1], ctypes.byref(ms)), 'timing')
+            return ms.value / 1e3 / reps
+
+        one = timed(1)
+        reps = int(min(50, max(3, budget_s / max(one, 1e-7))))
+        return min(timed(reps) for _ in range(3))
+    finally:
+        torch.cuda.synchronize()
+        for d, t, st in saved:
+            buf = d._dev_vals if isinstance(d, Mat) else d._dev
+            buf.copy_(t)
+            if st is not None:
+                d._state = st
+        torch.cuda.synchronize()
+        for e in ev:
+            nat.lib.op2hip_event_destroy(e)
 
 
 def par_loop(kernel, it_space, *args):
