@@ -85,11 +85,13 @@ def test_tuned_airfoil_matches_oracle(hip, tmp_path, monkeypatch):
     mesh = meshes.QuadMesh(48, 24, seed=3)
     with cfg.override(hip_autotune=True):
         A = airfoil.Airfoil(hip, mesh)
-        A.iteration()
-        q, rms = A.p_q.data_ro.copy(), np.array(A.rms.data).copy()
-        A.remove_consts()
-    B = airfoil.Airfoil(seq, mesh, consts=False)
-    B.iteration()
+        try:
+            A.iteration()
+            q, rms = A.p_q.data_ro.copy(), np.array(A.rms.data).copy()
+            B = airfoil.Airfoil(seq, mesh, consts=False)
+            B.iteration()
+        finally:
+            A.remove_consts()
     assert np.array_equal(q, B.p_q.data_ro)
     assert abs(rms[0] - B.rms.data[0]) <= 1e-12 * abs(B.rms.data[0])
     t = autotune.table()
