@@ -44,6 +44,8 @@ def parse():
     p.add_argument('--block-size', type=int, default=0)
     p.add_argument('--autotune', type=int, default=0,
                    help='tune launch parameters per loop and mesh-size bucket (1: on)')
+    p.add_argument('--owner-stage', type=int, default=-1,
+                   help='owner blocks stage owned rows of gathered Dats in LDS (-1: default)')
     p.add_argument('--owner-block-size', type=int, default=0)
     p.add_argument('--stream-unroll', type=int, default=0,
                    help='elements per thread of direct loops (0: the default)')
@@ -104,6 +106,8 @@ def main():
         opts['hip_stage_writes'] = bool(args.stage_writes)
     if args.layout_order:
         opts['hip_layout_order'] = args.layout_order
+    if args.owner_stage >= 0:
+        opts['hip_owner_stage'] = bool(args.owner_stage)
     if args.autotune:
         opts['hip_autotune'] = True
     if args.stream_unroll:

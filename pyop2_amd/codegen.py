@@ -149,7 +149,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
 '''
 
     def __init__(self, kernel, extents, args, consts, mode, block_size, prefetch=1,
-                 stage_reads=True, stage_writes=True, unroll=1):
+                 stage_reads=True, stage_writes=True, unroll=1, owner_stage=()):
         assert mode in ('stream', 'coloured', 'owner')
         # owner write-back: 1 issues the loads locating a thread's first
         # target before the elements run (they overlap), 0 after them
@@ -158,9 +158,17 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         self.stage_writes = stage_writes
         # elements per thread and chunk of a direct loop (their loads overlap)
         self.unroll = unroll
+        # owner loops: positions of read-only FP64 Dats on the increment
+        # target's layout whose owned rows a block stages in LDS
+        self.owner_stage = tuple(owner_stage) if mode == 'owner' else ()
         self.kernel = kernel
         self.extents = tuple(extents)
         self.infos = [ArgInfo(k, a, self.extents) for k, a in enumerate(args)]
+        # arguments naming one Dat share its staged rows (the first one's)
+        self._stage_of = {}
+        for q in self.owner_stage:
+            d = self.infos[q].arg.data
+            self._stage_of[q] = next(r for r in self.owner_stage if self.infos[r].arg.data is d)
         self.consts = consts
         self.mode = mode
         self.bs = block_size
@@ -313,6 +321,15 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                     decls.append('%s %s[%d];' % (t, n('inc'), a.cdim))
                     decls.append('for (int k = 0; k < %d; ++k) %s[k] = (%s)0;' % (a.cdim, n('inc'), t))
                     call_args.append(n('inc'))
+                elif local and p in self.owner_stage:
+                    # rows of the block's owned range come from LDS
+                    decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
+                    loads.append('{ const int tt = m%d[%s * %d + %d]; const %s* src = '
+                                 '(unsigned)(tt - t0) < (unsigned)nn_own ? s_st%d + (tt - t0) * %d : '
+                                 'd%d + (op2_i64)tt * %d; for (int k = 0; k < %d; ++k) %s[k] = src[k]; }'
+                                 % (p, E, a.mdim, a.idx, t, self._stage_of[p], a.cdim, p, a.cdim,
+                                    a.cdim, n('rl')))
+                    call_args.append(n('rl'))
                 elif local:
                     decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
                     loads.append('{ const %s* src = d%d + (op2_i64)m%d[%s * %d + %d] * %d; '
@@ -600,6 +617,20 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   const int t0 = p_t0[b];
   const bool dense = ng == p_t1[b] - t0;
   const int pitch = (int)lds_pitch;''' % sub]
+        if self.owner_stage:
+            out.append('  // the owned rows of read-only Dats on the target layout, staged')
+            out.append('  const int nn_own = p_t1[b] - t0, kmax = (int)own_k;')
+            off = '(op2_i64)%d * pitch' % (nac * cdim)
+            for pos in self.owner_stage:
+                if self._stage_of[pos] != pos:
+                    continue
+                a = self.infos[pos]
+                out.append('  %s* s_st%d = (%s*)(op2_lds + %s) + (op2_i64)kmax * %d;'
+                           % (a.ctype, pos, a.ctype, off, self._stage_offset(pos)))
+                out.append('  for (int i = threadIdx.x; i < nn_own * %d; i += %d) '
+                           's_st%d[i] = d%d[(op2_i64)t0 * %d + i];'
+                           % (a.cdim, bs, pos, pos, a.cdim))
+            out.append('  __syncthreads();')
         if pf >= 1:
             out.append('''  // write-back work of this thread's first owned target (its list entry
   // and contribution range), loaded while the elements run; the index is
@@ -754,13 +785,24 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                ind(self._red_finish('b'), 2)]
         return out
 
+    def _stage_offset(self, pos):
+        """Values per owned row staged before argument ``pos`` (all of
+        them for None)."""
+        n = 0
+        for q in self.owner_stage:
+            if q == pos:
+                break
+            if self._stage_of[q] == q:
+                n += self.infos[q].cdim
+        return n
+
     def owner_lds_bytes(self, k, max_exec, max_codes):
         """LDS bytes of an owner block: its elements' increments (Dat), or
         (Mat) its contributions' local-matrix rows, its elements'
         row-position table and its rows' CSR offsets."""
         a = self.owner_columns()[0][0]
         if a.kind != 'mat':
-            return max_exec * self.owner_slot_bytes()
+            return max_exec * self.owner_slot_bytes() + 8 * k * self._stage_offset(None)
         ent = self.extents[1] * a.rdim * a.cdim_m
         return 8 * ent * max(max_codes, 1) + 4 * a.rarity * max_exec + 4 * (k * a.rdim + 1)
 

@@ -30,6 +30,7 @@ DEFAULTS = {
     'hip_owner_block_size': 512,  # workgroup size of owner-computes loops (0: hip_block_size)
     'hip_stage_reads': False,     # stage read-only multi-component direct Dats through LDS
     'hip_stage_writes': True,     # stage written ones through LDS (else whole-row vector stores)
+    'hip_owner_stage': False,     # owner blocks stage their owned rows of gathered Dats in LDS
     'hip_owner_prefetch': 1,      # owner write-back lookups issued before the elements run (0/1)
     'hip_layout': True,           # locality renumbering of device storage
     'hip_layout_order': 'bisect',  # adjacency order of sets no map orders: 'bisect' or 'bfs'

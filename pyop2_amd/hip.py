@@ -1024,17 +1024,38 @@ class ParLoop(base.ParLoop):
                 return 'coloured'
         return 'owner'
 
+    def _owner_stage(self):
+        """Arguments an owner loop may stage in LDS: read-only FP64 Dats
+        gathered through a single map column whose Set shares the layout of
+        the increment target (so the block's owned positions are theirs)."""
+        args = self.args
+        race = [a for a in args if a._is_indirect and a.access is INC]
+        if not race or any(a._is_mat for a in args):
+            return ()
+        tgroup = layout._group(race[0].data.dataset)
+        written = set(id(a.data) for a in args if not a._is_global and a.access is not READ)
+        out = []
+        for k, a in enumerate(args):
+            if (a._is_indirect and a.access is READ and not a._is_vec_map and
+                    not isinstance(a.idx, IterationIndex) and a.data.dtype == np.float64 and
+                    id(a.data) not in written and a.data.cdim <= 16 and
+                    layout._group(a.data.dataset) is tgroup):
+                out.append(k)
+        return tuple(out)
+
     def _compiled(self, mode, bs, consts):
         key = base.JITModule._cache_key(self.kernel, self._it_space.extents, *self.args)
         pf = int(cfg.get('hip_owner_prefetch', 1))
         sr = bool(cfg.get('hip_stage_reads', False))
         sw = bool(cfg.get('hip_stage_writes', True))
         un = int(cfg.get('hip_stream_unroll', 1))
-        key = key + (mode, bs, pf, sr, sw, un)
+        st = self._owner_stage() if mode == 'owner' and cfg.get('hip_owner_stage') else ()
+        key = key + (mode, bs, pf, sr, sw, un, st)
         c = _loop_cache.get(key)
         if c is None:
             src = LoopSource(self.kernel, self._it_space.extents, self.args, consts, mode, bs,
-                             prefetch=pf, stage_reads=sr, stage_writes=sw, unroll=un)
+                             prefetch=pf, stage_reads=sr, stage_writes=sw, unroll=un,
+                             owner_stage=st)
             c = _CompiledLoop(src)
             _loop_cache[key] = c
         return c
@@ -1070,7 +1091,7 @@ class ParLoop(base.ParLoop):
 
 
 # configuration entries that shape a loop's launches (part of the call key)
-_VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill',
+_VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill', 'hip_owner_stage',
                  'hip_owner_targets', 'hip_owner', 'hip_partition_size', 'hip_reorder',
                  'hip_stage_reads', 'hip_stage_writes', 'hip_stream_unroll',
                  'hip_stream_blocks', 'hip_owner_prefetch')

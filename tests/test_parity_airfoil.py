@@ -95,3 +95,24 @@ def test_owner_mode_bit_exact(pair):
         W.update()
     assert np.array_equal(A.p_q.data_ro, B.p_q.data_ro)
     assert abs(A.rms.data[0] - B.rms.data[0]) <= RTOL * abs(B.rms.data[0])
+
+
+@pytest.mark.parametrize('opts', [{'hip_owner_stage': True},
+                                  {'hip_owner_block_size': 256, 'hip_owner_fill': 0.6},
+                                  {'hip_owner_prefetch': 0}])
+def test_owner_variants_bit_exact(pair, opts):
+    """Launch variants of the owner template (owned rows of q and adt staged
+    in LDS, other block sizes and owned ranges, no prefetch) keep the
+    element-ordered sums: res and q stay bit-exact to the oracle."""
+    from pyop2_amd import configuration as cfg
+    A, B = pair
+    with cfg.override(**opts):
+        for W in (A, B):
+            W.save_soln()
+            W.adt_calc()
+            W.res_calc()
+            W.bres_calc()
+        assert np.array_equal(A.p_res.data_ro, B.p_res.data_ro)
+        for W in (A, B):
+            W.update()
+    assert np.array_equal(A.p_q.data_ro, B.p_q.data_ro)
