@@ -38,6 +38,8 @@ def parse():
     p.add_argument('--nx', type=int, default=1200)
     p.add_argument('--ny', type=int, default=600)
     p.add_argument('--graph', type=int, default=1, help='replay each step as a hipGraph')
+    p.add_argument('--fuse', type=int, default=0,
+                   help='fuse consecutive owner loops into one Dat in the captured step')
     p.add_argument('--cpu-seconds', type=float, default=15.0,
                    help='approximate CPU time budget of the oracle baseline (0: skip)')
     p.add_argument('--partition-size', type=int, default=0)
@@ -132,7 +134,7 @@ def main():
     step = A.iteration
     graph = None
     if args.graph:
-        graph = hip.capture(A.iteration)
+        graph = hip.capture(A.iteration, fuse=bool(args.fuse))
         graph.replay()
         torch.cuda.synchronize()
         step = graph.replay
@@ -159,7 +161,7 @@ def main():
     # res_calc alone: a graph of `nres` back-to-back res_calc loops, HIP
     # events on the stream around its replays (no host time in between)
     nres = 10
-    rgraph = hip.capture(lambda: [A.res_calc() for _ in range(nres)])
+    rgraph = hip.capture(lambda: [A.res_calc() for _ in range(nres)], fuse=False)
     rgraph.replay()
     torch.cuda.synchronize()
     a, b = mkev(), mkev()
@@ -194,6 +196,7 @@ def main():
                    'unique_bytes_per_step': int(step_bytes),
                    'pct_of_hbm_peak': round(100.0 * value / HBM_PEAK_GBS, 2),
                    'graph': bool(args.graph),
+                   'fused_loops': bool(args.graph and args.fuse),
                    'autotune': _tuned() if args.autotune else False,
                    'partition_size': hip.cfg.get('hip_partition_size') or 'auto'},
         'roofline': {'bound': 'hbm', 'kernel': 'op2_res_calc_owner (one res_calc par_loop)',

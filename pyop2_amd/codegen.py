@@ -263,7 +263,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             lines.append('}')
         return lines
 
-    def _element_body(self, sfx='', ev='e', split=False):
+    def _element_body(self, sfx='', ev='e', split=False, infos=None, kernel=None):
         """Code that runs the kernel for element ``ev`` (all itspace points),
         leaving INC increments and Mat entries in private arrays; private
         names carry the suffix ``sfx``.
@@ -277,10 +277,12 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         kernels run."""
         decls, loads, compute, call_args, zeros, post = [], [], [], [], [], []
         ext = self.extents
-        written = set(id(a.arg.data) for a in self.infos
+        infos = self.infos if infos is None else infos
+        kernel = self.kernel if kernel is None else kernel
+        written = set(id(a.arg.data) for a in infos
                       if a.kind not in ('gbl_read', 'gbl_red') and a.access is not READ)
         E = '(op2_i64)%s' % ev
-        for a in self.infos:
+        for a in infos:
             p, t = a.pos, a.ctype
             n = lambda base: '%s%d%s' % (base, p, sfx)
             # read-only rows of Dats no argument writes go through registers
@@ -383,7 +385,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         call_args += ['i_%d' % k for k in range(len(ext))]
         opens = ['for (int i_%d = 0; i_%d < %d; ++i_%d) {' % (k, k, e, k)
                  for k, e in enumerate(ext)]
-        mats = [a for a in self.infos if a.kind == 'mat']
+        mats = [a for a in infos if a.kind == 'mat']
         if mats and len(ext) < 2:
             # Mat locals are indexed by both itspace indices
             if len(ext) == 0:
@@ -395,7 +397,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             closing = []
         compute += opens
         compute += zeros
-        compute.append('%s(%s);' % (self.kernel.name, ', '.join(call_args)))
+        compute.append('%s(%s);' % (kernel.name, ', '.join(call_args)))
         compute += ['}'] * len(opens)
         compute += closing
         compute += post
@@ -923,3 +925,162 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             lines.append('  }')
         lines.append('}')
         return '\n'.join(lines)
+
+
+class ChainSource(LoopSource):
+    """One owner-computes kernel running several consecutive loops that
+    increment the same Dat (and touch it no other way): a block runs the
+    elements of every loop that name its targets, keeps their increments in
+    LDS, and sums each owned target's increments loop by loop, each loop's
+    in ascending user element order, which is the order the loops reach
+    the target one after the other.  ``parts`` is a list of (kernel, args);
+    the loops have no iteration space, Mats or Global reductions."""
+
+    def __init__(self, parts, consts, block_size, prefetch=1):
+        args = [a for _, pargs in parts for a in pargs]
+        LoopSource.__init__(self, parts[0][0], (), args, consts, 'owner', block_size,
+                            prefetch=prefetch)
+        self.parts = []
+        off = 0
+        for kernel, pargs in parts:
+            self.parts.append((kernel, self.infos[off:off + len(pargs)]))
+            off += len(pargs)
+        self.entry = 'op2_%s_chain' % '_'.join(k.name for k, _ in parts)
+        self.reduce_entry = None
+        # optional occupancy request (waves per SIMD) for the register budget
+        self.min_waves = 0
+
+    def part_columns(self, j):
+        """owner_columns of part ``j`` alone (its race columns from 0)."""
+        out, ac = [], 0
+        for a in self.parts[j][1]:
+            if not a.race:
+                continue
+            n = 1 if a.kind == 'ind' else (a.mdim if a.kind == 'vec' else a.extent)
+            out.append((a, ac, n))
+            ac += n
+        return out
+
+    def nac(self):
+        return max(sum(n for _, _, n in self.part_columns(j)) for j in range(len(self.parts)))
+
+    def owner_slot_bytes(self):
+        a = self.part_columns(0)[0][0]
+        return self.nac() * a.cdim * np.dtype(a.arg.data.dtype).itemsize
+
+    def owner_lds_bytes(self, k, max_exec, max_codes):
+        return max_exec * self.owner_slot_bytes()
+
+    def param_roles(self):
+        roles = [('start', None), ('count', None), ('own_t0', None), ('own_t1', None)]
+        for j in range(len(self.parts)):
+            roles += [('own_rlo%d' % j, None), ('own_rn%d' % j, None), ('own_exoff%d' % j, None),
+                      ('own_exec%d' % j, None)]
+        roles += [('own_tgoff', None), ('own_tgt', None), ('own_tgcb', None), ('own_tp', None),
+                  ('own_contrib', None), ('own_c16', None), ('own_pitch', None)]
+        for a in self.infos:
+            for _, role in a.params():
+                roles.append((role, a.pos))
+        return roles
+
+    def _params_decl(self):
+        decl = ['op2_i64 start', 'op2_i64 count', 'const int* __restrict__ p_t0',
+                'const int* __restrict__ p_t1']
+        for j in range(len(self.parts)):
+            decl += ['const int* __restrict__ p_rlo%d' % j, 'const int* __restrict__ p_rn%d' % j,
+                     'const int* __restrict__ p_exoff%d' % j, 'const int* __restrict__ p_exec%d' % j]
+        decl += ['const int* __restrict__ p_tgoff', 'const int* __restrict__ p_tgt',
+                 'const int* __restrict__ p_tgcb', 'const int* __restrict__ p_tp',
+                 'const void* __restrict__ p_contrib', 'op2_i64 own_c16', 'op2_i64 lds_pitch']
+        for a in self.infos:
+            decl += [d for d, _ in a.params()]
+        return decl
+
+    def source(self):
+        ind = lambda lines, n: '\n'.join(' ' * n + l for l in lines)
+        bs = self.bs
+        out = [self.PRELUDE, self._const_decls(), self._row_types()]
+        seen = set()
+        for kernel, _ in self.parts:
+            if kernel.name in seen:
+                continue
+            seen.add(kernel.name)
+            out += ['#pragma clang force_cuda_host_device begin', 'inline', kernel.code,
+                    '#pragma clang force_cuda_host_device end', '']
+        bounds = '%d, %d' % (bs, self.min_waves) if self.min_waves else '%d' % bs
+        out.append('extern "C" __global__ void __launch_bounds__(%s) %s(\n    %s) {'
+                   % (bounds, self.entry, ',\n    '.join(self._params_decl())))
+        tgt = self.part_columns(0)[0][0]
+        t, cdim = tgt.ctype, tgt.cdim
+        nac = self.nac()
+        sub = {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos, 'nac': nac, 'cmax': OWNER_CMAX}
+        out.append('''  extern __shared__ %(t)s op2_lds[];
+  const int nb = (int)count, bid = blockIdx.x;
+  const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
+  const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
+  const int g0 = p_tgoff[b], ng = p_tgoff[b + 1] - g0;
+  const int t0 = p_t0[b];
+  const bool dense = ng == p_t1[b] - t0;
+  const int pitch = (int)lds_pitch;''' % sub)
+        # slots: part 0's elements first, then part 1's, ...
+        out.append('  int nex = 0;')
+        for j in range(len(self.parts)):
+            out.append('  const int rlo%d = p_rlo%d[b], rn%d = p_rn%d[b], ex%d = p_exoff%d[b];'
+                       % (j, j, j, j, j, j))
+            out.append('  const int sb%d = nex, ne%d = rn%d + p_exoff%d[b + 1] - ex%d;'
+                       % (j, j, j, j, j))
+            out.append('  nex += ne%d;' % j)
+        out.append('  for (int l = threadIdx.x; l < nex; l += %d) {' % bs)
+        for j, (kernel, infos) in enumerate(self.parts):
+            decls, compute = self._element_body(infos=infos, kernel=kernel)
+            store = []
+            for a, ac0, n in self.part_columns(j):
+                if a.kind == 'ind':
+                    store.append('for (int k = 0; k < %d; ++k) op2_lds[(%d + k) * pitch + l] = '
+                                 'inc%d[k];' % (cdim, ac0 * cdim, a.pos))
+                else:
+                    store.append('for (int j = 0; j < %d; ++j) for (int k = 0; k < %d; ++k) '
+                                 'op2_lds[((%d + j) * %d + k) * pitch + l] = inc%d[j][k];'
+                                 % (n, cdim, ac0, cdim, a.pos))
+            if j == 0:
+                out.append('    if (l < sb0 + ne0) {')
+            elif j < len(self.parts) - 1:
+                out.append('    else if (l < sb%d + ne%d) {' % (j, j))
+            else:
+                out.append('    else {')
+            out.append('      const int lj = l - sb%d;' % j)
+            out.append('      const unsigned code = lj < rn%d ? (unsigned)(rlo%d + lj) : '
+                       '(unsigned)p_exec%d[ex%d + lj - rn%d];' % (j, j, j, j, j))
+            out.append('      const int e = (int)(code & 0x7fffffffu);')
+            out.append(ind(decls + compute + store, 6))
+            out.append('    }')
+        out.append('  }')
+        out.append('  __syncthreads();')
+        out.append('''  for (int j = threadIdx.x; j < ng; j += %(bs)d) {
+    const int w_t = dense ? t0 + j : p_tgt[g0 + j];
+    const int w_cb = p_tgcb[g0 + j], w_nc = p_tgcb[g0 + j + 1] - w_cb;
+    int w_code[%(cmax)d];
+#pragma unroll
+    for (int u = 0; u < %(cmax)d; ++u) w_code[u] = u < w_nc ? OP2_CONTRIB(w_cb + u) : 0;
+    %(t)s acc[%(cdim)d];
+#pragma unroll
+    for (int k = 0; k < %(cdim)d; ++k) acc[k] = d%(pos)d[(op2_i64)w_t * %(cdim)d + k];
+#pragma unroll
+    for (int u = 0; u < %(cmax)d; ++u) {
+      if (u < w_nc) {
+        const int slot = w_code[u] / %(nac)d, ac = w_code[u] - slot * %(nac)d;
+#pragma unroll
+        for (int k = 0; k < %(cdim)d; ++k) acc[k] += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
+      }
+    }
+    for (int c = %(cmax)d; c < w_nc; ++c) {
+      const int cc = OP2_CONTRIB(w_cb + c);
+      const int slot = cc / %(nac)d, ac = cc - slot * %(nac)d;
+#pragma unroll
+      for (int k = 0; k < %(cdim)d; ++k) acc[k] += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
+    }
+#pragma unroll
+    for (int k = 0; k < %(cdim)d; ++k) d%(pos)d[(op2_i64)w_t * %(cdim)d + k] = acc[k];
+  }
+}''' % sub)
+        return '\n'.join(out)

@@ -31,6 +31,8 @@ DEFAULTS = {
     'hip_stage_reads': False,     # stage read-only multi-component direct Dats through LDS
     'hip_stage_writes': True,     # stage written ones through LDS (else whole-row vector stores)
     'hip_owner_stage': False,     # owner blocks stage their owned rows of gathered Dats in LDS
+    'hip_chain': True,            # fuse consecutive owner loops into one Dat in fusing captures
+    'hip_chain_waves': 8,         # waves per SIMD a chain kernel's registers are budgeted for
     'hip_owner_prefetch': 1,      # owner write-back lookups issued before the elements run (0/1)
     'hip_layout': True,           # locality renumbering of device storage
     'hip_layout_order': 'bisect',  # adjacency order of sets no map orders: 'bisect' or 'bfs'
