@@ -1058,7 +1058,7 @@ class ChainSource(LoopSource):
         out.append('  __syncthreads();')
         out.append('''  for (int j = threadIdx.x; j < ng; j += %(bs)d) {
     const int w_t = dense ? t0 + j : p_tgt[g0 + j];
-    const int w_cb = p_tgcb[g0 + j], w_nc = p_tgcb[g0 + j + 1] - w_cb;
+    const int w_cb = p_tp[w_t], w_nc = p_tp[w_t + 1] - w_cb;
     int w_code[%(cmax)d];
 #pragma unroll
     for (int u = 0; u < %(cmax)d; ++u) w_code[u] = u < w_nc ? OP2_CONTRIB(w_cb + u) : 0;

@@ -1124,6 +1124,22 @@ def owner_plan_for(iterset, n, args, k):
     return p
 
 
+def _xcd_front_order(front):
+    """Block order moving the blocks flagged in ``front`` to the start of
+    the run of blocks each XCD executes (the owner kernels give XCD x the
+    x-th of 8 contiguous runs), keeping the order otherwise."""
+    nb = len(front)
+    q, rr = nb // 8, nb % 8
+    order = []
+    start = 0
+    for x in range(8):
+        n = q + (1 if x < rr else 0)
+        idx = np.arange(start, start + n)
+        order.append(np.concatenate([idx[front[idx]], idx[~front[idx]]]))
+        start += n
+    return np.concatenate(order).astype(np.int64)
+
+
 class ChainPlan(object):
     """Owner plans of consecutive loops over one partition of the targets
     (same owned range ``k``) merged into one: a block's slots hold loop 0's
@@ -1181,7 +1197,32 @@ class ChainPlan(object):
             self.tg_off[-1] = np.searchsorted(touched, self.blk_t1[-1])
         self.tg_cb = np.append(self.tp[touched], self.tp[-1]).astype(np.int32)
         self.c16 = self.max_exec * self.nac <= 0xffff
+        # blocks that run elements of later loops take longer: start them
+        # first within their XCD's run of blocks, so they do not finish last
+        heavy = (nex[1:] > 0).any(axis=0) if len(plans) > 1 else np.zeros(nb, dtype=bool)
+        self._permute(_xcd_front_order(heavy))
         self._dev = None
+
+    def _permute(self, order):
+        """Reorder the blocks (per-block arrays and the per-block segments
+        of the element and touched-target lists)."""
+        def segs(off, arr):
+            parts = [arr[off[b]:off[b + 1]] for b in order]
+            new = np.concatenate(parts) if parts else arr[:0]
+            noff = np.concatenate([[0], np.cumsum([len(x) for x in parts])]).astype(np.int32)
+            return noff, new
+        self.blk_t0, self.blk_t1 = self.blk_t0[order], self.blk_t1[order]
+        for part in self.parts:
+            part['rng_lo'], part['rng_n'] = part['rng_lo'][order], part['rng_n'][order]
+            part['ex_off'], part['exec'] = segs(part['ex_off'], part['exec'])
+        g = self.tg_off
+        tcb = self.tg_cb[:-1]
+        parts_t = [self.tgt[g[b]:g[b + 1]] for b in order]
+        parts_c = [tcb[g[b]:g[b + 1]] for b in order]
+        self.tgt = np.concatenate(parts_t).astype(np.int32) if parts_t else self.tgt
+        self.tg_cb = np.append(np.concatenate(parts_c) if parts_c else tcb,
+                               self.tg_cb[-1]).astype(np.int32)
+        self.tg_off = np.concatenate([[0], np.cumsum([len(x) for x in parts_t])]).astype(np.int32)
 
     def device_arrays(self):
         if self._dev is None:
