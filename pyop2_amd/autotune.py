@@ -25,6 +25,8 @@ CANDIDATES = {
     'stream': [{'hip_block_size': b} for b in (128, 256, 512)],
     'owner': [{'hip_owner_block_size': b, 'hip_owner_fill': f}
               for b in (256, 512) for f in (0.6, 0.9, 1.2)],
+    'owner_mat': [{'hip_owner_mat_block_size': b, 'hip_owner_mat_fill': f}
+                  for b in (128, 256, 512) for f in (0.4, 0.6, 0.9)],
     'coloured': [{'hip_partition_size': p, 'hip_block_size': b}
                  for p in (128, 256, 512, 1024) for b in (128, 256) if b <= p],
 }
@@ -82,7 +84,10 @@ def params_for(pl, mode, trial):
     if key in t:
         return t[key]['params']
     best, best_s, tried = None, None, []
-    for cand in CANDIDATES.get(mode, [{}]):
+    kind = mode
+    if mode == 'owner' and any(a._is_mat for a in pl.args):
+        kind = 'owner_mat'
+    for cand in CANDIDATES.get(kind, [{}]):
         try:
             sec = trial(cand)
         except Exception as e:   # a candidate that cannot run (LDS, plan) is skipped

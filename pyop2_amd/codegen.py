@@ -739,6 +739,32 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   for (int i = threadIdx.x; i < nex * %(nac)d; i += %(bs)d) s_inv[i] = -1;
   for (int i = threadIdx.x; i <= nn * %(rd)d; i += %(bs)d) s_rp[i] = rp%(p)d[t0 * %(rd)d + i];
   __syncthreads();
+  // the rows' CSR slots of pair pp: last owned node whose first pair <= pp
+#define OP2_PAIR_ROWS(pp, sl) { \\
+    int lo = 0, hi = nn - 1; \\
+    while (lo < hi) { \\
+      const int mid = (lo + hi + 1) >> 1; \\
+      if (s_rp[mid * %(rd)d] / %(rdcd)d <= (pp)) lo = mid; else hi = mid - 1; \\
+    } \\
+    const int kcol = (pp) - s_rp[lo * %(rd)d] / %(rdcd)d; \\
+    for (int r = 0; r < %(rd)d; ++r) sl[r] = s_rp[lo * %(rd)d + r] + kcol * %(cd)d; }
+  const int pp0 = s_rp[0] / %(rdcd)d, pp1 = s_rp[nn * %(rd)d] / %(rdcd)d;
+  // this thread's first pair: slots, program range and old values, loaded
+  // now so they arrive while the elements run
+  int f_sl[%(rd)d], f_q0 = 0, f_q1 = 0;
+  double f_acc[%(rd)d][%(cd)d];
+  {
+    const int pp = pp0 + (int)threadIdx.x < pp1 ? pp0 + (int)threadIdx.x : pp0;
+    OP2_PAIR_ROWS(pp, f_sl);
+    if (pp0 < pp1) {
+      f_q0 = p_poff[pp];
+      f_q1 = p_poff[pp + 1];
+#pragma unroll
+      for (int r = 0; r < %(rd)d; ++r)
+#pragma unroll
+        for (int c = 0; c < %(cd)d; ++c) f_acc[r][c] = d%(p)d[f_sl[r] + c];
+    }
+  }
   // a contribution code is slot * %(nac)d + row position: it indexes s_inv
   for (int i = threadIdx.x; i < ncode; i += %(bs)d) s_inv[OP2_CONTRIB(c0 + i)] = i;
   __syncthreads();
@@ -750,40 +776,55 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                ind(decls + compute + store, 4),
                '  }',
                '  __syncthreads();',
-               '''  // one thread per node pair of the owned rows; pair pp of node n starts
-  // at rowptr[n * %(rd)d] / %(rdcd)d
-  const int nrow = nn * %(rd)d;
-  const int pp1 = s_rp[nrow] / %(rdcd)d;
-  for (int pp = s_rp[0] / %(rdcd)d + (int)threadIdx.x; pp < pp1; pp += %(bs)d) {
-    int lo = 0, hi = nn - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_rp[mid * %(rd)d] / %(rdcd)d <= pp) lo = mid; else hi = mid - 1;
-    }
-    const int kcol = pp - s_rp[lo * %(rd)d] / %(rdcd)d;
-    double acc[%(rd)d][%(cd)d];
+               '''  // one thread per node pair of the owned rows (pair pp of node n
+  // starts at rowptr[n * %(rd)d] / %(rdcd)d); the first pair's lookups and old
+  // values were loaded before the elements ran
+  for (int pp = pp0 + (int)threadIdx.x, it = 0; pp < pp1; pp += %(bs)d, ++it) {
     int sl[%(rd)d];
+    double acc[%(rd)d][%(cd)d];
+    int q0, q1;
+    if (it == 0) {
+      q0 = f_q0;
+      q1 = f_q1;
 #pragma unroll
-    for (int r = 0; r < %(rd)d; ++r) {
-      sl[r] = s_rp[lo * %(rd)d + r] + kcol * %(cd)d;
+      for (int r = 0; r < %(rd)d; ++r) {
+        sl[r] = f_sl[r];
 #pragma unroll
-      for (int c = 0; c < %(cd)d; ++c) acc[r][c] = d%(p)d[sl[r] + c];
-    }
-    const int q1 = p_poff[pp + 1];
-    for (int qq = p_poff[pp]; qq < q1; ++qq) {
-      const int code = own_wide ? ((const int*)p_prog)[qq] : (int)((const unsigned short*)p_prog)[qq];
-      const int cl = code / %(e1)d, j = code - cl * %(e1)d;
+        for (int c = 0; c < %(cd)d; ++c) acc[r][c] = f_acc[r][c];
+      }
+    } else {
+      OP2_PAIR_ROWS(pp, sl);
+      q0 = p_poff[pp];
+      q1 = p_poff[pp + 1];
 #pragma unroll
       for (int r = 0; r < %(rd)d; ++r)
 #pragma unroll
-        for (int c = 0; c < %(cd)d; ++c)
-          acc[r][c] %(op)s s_val[((j * %(rd)d + r) * %(cd)d + c) * cpitch + cl];
+        for (int c = 0; c < %(cd)d; ++c) acc[r][c] = d%(p)d[sl[r] + c];
+    }
+    // scatter-program codes, %(pmax)d loads at a time
+    for (int qb = q0; qb < q1; qb += %(pmax)d) {
+      int code[%(pmax)d];
+#pragma unroll
+      for (int u = 0; u < %(pmax)d; ++u)
+        code[u] = qb + u < q1 ? (own_wide ? ((const int*)p_prog)[qb + u]
+                                          : (int)((const unsigned short*)p_prog)[qb + u]) : 0;
+#pragma unroll
+      for (int u = 0; u < %(pmax)d; ++u) {
+        if (qb + u < q1) {
+          const int cl = code[u] / %(e1)d, j = code[u] - cl * %(e1)d;
+#pragma unroll
+          for (int r = 0; r < %(rd)d; ++r)
+#pragma unroll
+            for (int c = 0; c < %(cd)d; ++c)
+              acc[r][c] %(op)s s_val[((j * %(rd)d + r) * %(cd)d + c) * cpitch + cl];
+        }
+      }
     }
 #pragma unroll
     for (int r = 0; r < %(rd)d; ++r)
 #pragma unroll
       for (int c = 0; c < %(cd)d; ++c) d%(p)d[sl[r] + c] = acc[r][c];
-  }''' % sub,
+  }''' % dict(sub, pmax=8),
                ind(self._red_finish('b'), 2)]
         return out
 

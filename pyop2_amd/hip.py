@@ -1069,9 +1069,17 @@ class ParLoop(base.ParLoop):
         """A launch entry for ``mode`` under the current configuration
         (owner loops whose plan does not fit fall back to colouring)."""
         bs = _block_size()
-        obs = cfg.get('hip_owner_block_size') or bs
+        if mode == 'owner' and any(a._is_mat for a in self.args):
+            # Mat assembly keeps a local-matrix row per contribution in LDS:
+            # smaller blocks and owned ranges hide its latency better
+            obs = cfg.get('hip_owner_mat_block_size') or bs
+            fill = cfg.get('hip_owner_mat_fill') or cfg.get('hip_owner_fill')
+        else:
+            obs = cfg.get('hip_owner_block_size') or bs
+            fill = cfg.get('hip_owner_fill')
         try:
-            return _LaunchEntry(self, mode, obs if mode == 'owner' else bs)
+            with cfg.override(hip_owner_fill=fill):
+                return _LaunchEntry(self, mode, obs if mode == 'owner' else bs)
         except _NoOwnerPlan:
             return _LaunchEntry(self, 'coloured', bs)
 
@@ -1416,6 +1424,7 @@ def run_chained(loops, launch=True):
 
 # configuration entries that shape a loop's launches (part of the call key)
 _VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill', 'hip_owner_stage',
+                 'hip_owner_mat_block_size', 'hip_owner_mat_fill',
                  'hip_owner_targets', 'hip_owner', 'hip_partition_size', 'hip_reorder',
                  'hip_stage_reads', 'hip_stage_writes', 'hip_stream_unroll',
                  'hip_stream_blocks', 'hip_owner_prefetch')
