@@ -58,6 +58,14 @@ int op2hip_module_function(void* mod, const char* name, void** fn_out);
 int op2hip_module_global(void* mod, const char* name, void** dptr, size_t* bytes);
 /* Copy `bytes` host bytes to a module global on `stream`, then wait. */
 int op2hip_set_global(void* dptr, const void* src, size_t bytes, void* stream);
+/* Compile (hiprtc, cached), load and resolve `entry` in one call; the
+ * module belongs to the caller.  The JITModule compile+lookup of
+ * pyop2/cuda.py:602-651 as one entry point. */
+int op2hip_module_compile(const char* src, const char* entry, const char* const* opts, int nopts,
+                          const char* cache_dir, void** mod_out, void** fn_out);
+/* Set a module global (a Const, pyop2/base.py:857-926) by name. */
+int op2hip_module_set_const(void* mod, const char* name, const void* host, size_t bytes,
+                            void* stream);
 
 /* Launch a generated loop kernel whose parameters are all 8 bytes wide:
  * argv[i] is parameter i (pointer or 64-bit integer).
@@ -210,6 +218,12 @@ int op2hip_cgb_read(void* h, op2hip_cg_system* out, void* stream);
 int op2hip_cgb_history(void* h, int k, double* out, void* stream);
 int op2hip_cgb_sizeof_system(void);
 int op2hip_cgb_chunk_rows(void);
+/* One solve (replaces Solver.solve, pyop2/petsc_base.py:182-220, for one
+ * CSR on the device): work holds 5*n device doubles; its, rnorm and reason
+ * (KSPConvergedReason) are host outputs.  Waits for the stream. */
+int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, const double* b,
+               double* x, double* work, double rtol, double atol, double dtol, int maxit,
+               int* its, double* rnorm, int* reason, void* stream);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,12 @@ _SIGS = {
                                    c_void, c_int, c_int, c_int, c_void, c_void, c_int, c_int,
                                    c_int, c_void, c_void]),
     'op2hip_order_bfs': (c_int, [c_int, c_int, c_void, c_int, c_void]),
+    'op2hip_module_compile': (c_int, [ctypes.c_char_p, ctypes.c_char_p, c_void, c_int,
+                                      ctypes.c_char_p, P(c_void), P(c_void)]),
+    'op2hip_module_set_const': (c_int, [c_void, ctypes.c_char_p, c_void, c_size, c_void]),
+    'op2hip_pcg': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void, ctypes.c_double,
+                           ctypes.c_double, ctypes.c_double, c_int, P(c_int),
+                           P(ctypes.c_double), P(c_int), c_void]),
     'op2hip_order_bisect': (c_int, [c_int, c_int, c_void, c_int, c_int, c_void]),
     'op2hip_order_first_touch': (c_int, [c_int, c_int, c_void, c_int, c_void, c_void]),
     'op2hip_sparsity_build': (c_int, [c_int, c_int, c_int, c_int, P(c_void), P(c_int), P(c_void),

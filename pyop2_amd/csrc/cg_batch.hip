@@ -434,4 +434,52 @@ int op2hip_cgb_sizeof_system(void) { return static_cast<int>(sizeof(Sys)); }
 
 int op2hip_cgb_chunk_rows(void) { return kChunk; }
 
+
+// One Jacobi-PCG solve (the single-system form of the batch): A (n x n CSR
+// on the device), b, x (device, x overwritten), work (device, 5*n doubles:
+// D^-1, r, z, p, w).  PETSc KSPCG + PCJACOBI semantics as above; its,
+// rnorm (the last preconditioned residual norm) and reason (KSP converged
+// reason) are written to host memory.  Waits for the stream.
+int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, const double* b,
+               double* x, double* work, double rtol, double atol, double dtol, int maxit,
+               int* its, double* rnorm, int* reason, void* stream) {
+  OP2HIP_REQUIRE(n >= 0 && rowptr && colidx && vals && b && x && work && its && rnorm && reason,
+                 "null argument to op2hip_pcg");
+  void* h = nullptr;
+  int st = op2hip_cgb_create(1, (n + kChunk - 1) / kChunk + 1, 0, &h);
+  if (st != OP2HIP_OK) return st;
+  Sys sy{};
+  sy.rowptr = rowptr;
+  sy.colidx = colidx;
+  sy.vals = vals;
+  sy.b = b;
+  sy.x = x;
+  sy.inv_diag = work;
+  sy.r = work + n;
+  sy.z = work + 2 * (size_t)n;
+  sy.p = work + 3 * (size_t)n;
+  sy.w = work + 4 * (size_t)n;
+  sy.row0 = 0;
+  sy.n = n;
+  sy.rtol = rtol;
+  sy.atol = atol;
+  sy.dtol = dtol;
+  sy.maxit = maxit;
+  sy.state = SYS_NEW;
+  st = op2hip_cgb_load(h, 1, &sy, stream);
+  for (int done = 0; st == OP2HIP_OK;) {
+    const int k = done ? 16 : 4;
+    st = op2hip_cgb_step(h, k, stream);
+    if (st == OP2HIP_OK) st = op2hip_cgb_read(h, &sy, stream);
+    done += k;
+    if (st != OP2HIP_OK || sy.state == SYS_DONE || done > maxit + 16) break;
+  }
+  op2hip_cgb_destroy(h);
+  if (st != OP2HIP_OK) return st;
+  *its = sy.its;
+  *rnorm = sy.dp;
+  *reason = sy.reason;
+  return OP2HIP_OK;
+}
+
 }  // extern "C"

@@ -244,6 +244,39 @@ int op2hip_set_global(void* dptr, const void* src, size_t bytes, void* stream) {
   return OP2HIP_OK;
 }
 
+// Compile, load and look up in one call: `src` through hiprtc (with the
+// code-object cache), its module loaded, `entry` resolved.  The module is
+// the caller's (op2hip_module_destroy).
+int op2hip_module_compile(const char* src, const char* entry, const char* const* opts, int nopts,
+                          const char* cache_dir, void** mod_out, void** fn_out) {
+  OP2HIP_REQUIRE(src && entry && mod_out && fn_out, "null argument to op2hip_module_compile");
+  void* code = nullptr;
+  size_t size = 0;
+  int hit = 0;
+  int st = op2hip_compile(src, opts, nopts, cache_dir, &code, &size, &hit);
+  if (st != OP2HIP_OK) return st;
+  st = op2hip_module_load(code, size, mod_out);
+  op2hip_free_host(code);
+  if (st != OP2HIP_OK) return st;
+  st = op2hip_module_function(*mod_out, entry, fn_out);
+  if (st != OP2HIP_OK) {
+    op2hip_module_destroy(*mod_out);
+    *mod_out = nullptr;
+  }
+  return st;
+}
+
+// Set a module global (a kernel Const) by name from `bytes` host bytes.
+int op2hip_module_set_const(void* mod, const char* name, const void* host, size_t bytes,
+                            void* stream) {
+  void* dptr = nullptr;
+  size_t size = 0;
+  int st = op2hip_module_global(mod, name, &dptr, &size);
+  if (st != OP2HIP_OK) return st;
+  OP2HIP_REQUIRE(bytes <= size, "Const %s holds %zu bytes, %zu given", name, size, bytes);
+  return op2hip_set_global(dptr, host, bytes, stream);
+}
+
 // Launch a generated loop kernel whose parameters are all 8 bytes wide
 // (pointers and 64-bit integers): argv[i] is the i-th parameter's value.
 int op2hip_launch(void* fn, unsigned grid, unsigned block, unsigned lds, void* stream,
