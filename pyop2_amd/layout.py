@@ -110,7 +110,20 @@ def _same(x, y):
 
 
 def register_map(m):
-    """Record a map on its sets (called when a hip Map is built)."""
+    """Record a map on its sets (called when a hip Map is built).  Two maps
+    from one Set with identical values into two Sets of one size (a mesh's
+    element-to-node map onto scalar, vector and coordinate node Sets) give
+    their target Sets one layout, so the maps' device copies are identical
+    too (one buffer, one gather pattern)."""
+    for other in getattr(m.iterset, '_maps_from', []):
+        if (other.dataset is not m.dataset and other.dim == m.dim and
+                other.dataset.total_size == m.dataset.total_size and
+                np.shape(other.values) == np.shape(m.values) and
+                np.array_equal(other.values, m.values)):
+            ga, gb = _group(other.dataset), _group(m.dataset)
+            if ga is not gb and (ga.layout is None or gb.layout is None):
+                link(other.dataset, m.dataset)
+            break
     for s, attr in ((m.iterset, '_maps_from'), (m.dataset, '_maps_to')):
         lst = getattr(s, attr, None)
         if lst is None:

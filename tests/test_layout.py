@@ -123,3 +123,16 @@ def test_components_ordered_as_alone(how):
     for k, m in enumerate(ms):
         alone = layout.adjacency_order(m.nvert, m.elem_node, how)
         assert np.array_equal(perm[off[k]:off[k + 1]] - off[k], alone)
+
+
+def test_identical_maps_share_target_layout():
+    """Element-to-node maps with identical values onto scalar and vector
+    node Sets put both Sets in one layout, so their device maps coincide."""
+    mesh = meshes.TriMesh(15, seed=2)
+    el = hip.Set(mesh.nelem, 1, 'el')
+    nodes = hip.Set(mesh.nvert, 1, 'nodes')
+    vnodes = hip.Set(mesh.nvert, 2, 'vnodes')
+    en = hip.Map(el, nodes, 3, mesh.elem_node, 'en')
+    ev = hip.Map(el, vnodes, 3, mesh.elem_node, 'ev')
+    assert np.array_equal(layout.of(nodes).perm, layout.of(vnodes).perm)
+    assert np.array_equal(layout.map_values(en), layout.map_values(ev))
