@@ -40,6 +40,8 @@ def parse():
     p.add_argument('--graph', type=int, default=1, help='replay each step as a hipGraph')
     p.add_argument('--fuse', type=int, default=0,
                    help='fuse consecutive owner loops into one Dat in the captured step')
+    p.add_argument('--chain-waves', type=int, default=-1,
+                   help='waves per SIMD a fused kernel is budgeted for (-1: default, 0: none)')
     p.add_argument('--cpu-seconds', type=float, default=15.0,
                    help='approximate CPU time budget of the oracle baseline (0: skip)')
     p.add_argument('--partition-size', type=int, default=0)
@@ -108,6 +110,8 @@ def main():
         opts['hip_stage_writes'] = bool(args.stage_writes)
     if args.layout_order:
         opts['hip_layout_order'] = args.layout_order
+    if args.chain_waves >= 0:
+        opts['hip_chain_waves'] = args.chain_waves
     if args.owner_stage >= 0:
         opts['hip_owner_stage'] = bool(args.owner_stage)
     if args.autotune:

@@ -1062,7 +1062,12 @@ class ChainSource(LoopSource):
   const int g0 = p_tgoff[b], ng = p_tgoff[b + 1] - g0;
   const int t0 = p_t0[b];
   const bool dense = ng == p_t1[b] - t0;
-  const int pitch = (int)lds_pitch;''' % sub)
+  const int pitch = (int)lds_pitch;
+  // this thread's first target and its contribution range, loaded while
+  // the elements run (index clamped to the padding entry of the list)
+  const int jf = (int)threadIdx.x < ng ? (int)threadIdx.x : ng;
+  const int f_t = dense ? t0 + jf : p_tgt[g0 + jf];
+  const int f_cb = p_tp[f_t], f_ce = p_tp[f_t + 1];''' % sub)
         # slots: part 0's elements first, then part 1's, ...
         out.append('  int nex = 0;')
         for j in range(len(self.parts)):
@@ -1098,8 +1103,16 @@ class ChainSource(LoopSource):
         out.append('  }')
         out.append('  __syncthreads();')
         out.append('''  for (int j = threadIdx.x; j < ng; j += %(bs)d) {
-    const int w_t = dense ? t0 + j : p_tgt[g0 + j];
-    const int w_cb = p_tp[w_t], w_nc = p_tp[w_t + 1] - w_cb;
+    int w_t, w_cb, w_nc;
+    if (j == (int)threadIdx.x) {
+      w_t = f_t;
+      w_cb = f_cb;
+      w_nc = f_ce - f_cb;
+    } else {
+      w_t = dense ? t0 + j : p_tgt[g0 + j];
+      w_cb = p_tp[w_t];
+      w_nc = p_tp[w_t + 1] - w_cb;
+    }
     int w_code[%(cmax)d];
 #pragma unroll
     for (int u = 0; u < %(cmax)d; ++u) w_code[u] = u < w_nc ? OP2_CONTRIB(w_cb + u) : 0;
