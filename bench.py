@@ -8,7 +8,9 @@ step x steps / wall time of the timed region, inputs resident in HBM.
 
 ``roofline`` is for res_calc, the dominant loop: its unique bytes per call
 divided by its average duration, timed with HIP events on the stream the
-loop is launched on.  ``cpu_baseline`` times the reference's OpenMP backend
+loop is launched on.  ``cache_cold`` times res_calc and adt_calc each
+after a 512 MiB write and read, so that their inputs (about 110 and 52 MB, which
+would fit the 256 MB Infinity Cache) come from HBM.  ``cpu_baseline`` times the reference's OpenMP backend
 restated (block colouring, the host's OpenMP threads), with the sequential
 oracle (one core) beside it, on a bounded number of steps of the same
 mesh.
@@ -28,6 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# bytes written between the launches of the cache-cold loop timings
+FLUSH_BYTES = 512 << 20
 
 
 def parse():
@@ -51,6 +55,10 @@ def parse():
     p.add_argument('--owner-stage', type=int, default=-1,
                    help='owner blocks stage owned rows of gathered Dats in LDS (-1: default)')
     p.add_argument('--owner-block-size', type=int, default=0)
+    p.add_argument('--owner-prefetch', type=int, default=-1,
+                   help='owner write-back loads issued before the elements (-1: default)')
+    p.add_argument('--owner-waves', type=int, default=-1,
+                   help='waves per SIMD owner loops are compiled for (-1: default, 0: none)')
     p.add_argument('--stream-unroll', type=int, default=0,
                    help='elements per thread of direct loops (0: the default)')
     p.add_argument('--stream-blocks', type=int, default=0,
@@ -110,6 +118,10 @@ def main():
         opts['hip_stage_writes'] = bool(args.stage_writes)
     if args.layout_order:
         opts['hip_layout_order'] = args.layout_order
+    if args.owner_prefetch >= 0:
+        opts['hip_owner_prefetch'] = args.owner_prefetch
+    if args.owner_waves >= 0:
+        opts['hip_owner_waves'] = args.owner_waves
     if args.chain_waves >= 0:
         opts['hip_chain_waves'] = args.chain_waves
     if args.owner_stage >= 0:
@@ -162,22 +174,57 @@ def main():
     gpu_s = ms.value / 1e3
     elapsed = max(wall, gpu_s)
 
-    # res_calc alone: a graph of `nres` back-to-back res_calc loops, HIP
-    # events on the stream around its replays (no host time in between)
-    nres = 10
-    rgraph = hip.capture(lambda: [A.res_calc() for _ in range(nres)], fuse=False)
-    rgraph.replay()
-    torch.cuda.synchronize()
-    a, b = mkev(), mkev()
-    reps = max(3, args.steps // 5)
-    nat.lib.op2hip_event_record(a, ctypes.c_void_p(s))
-    for _ in range(reps):
-        rgraph.replay()
-    nat.lib.op2hip_event_record(b, ctypes.c_void_p(s))
-    torch.cuda.synchronize()
-    nat.lib.op2hip_event_elapsed_ms(a, b, ctypes.byref(ms))
-    res_avg_s = ms.value / 1e3 / (reps * nres)
+    def hot_avg(loop, nrep=10):
+        # one loop alone: a graph of `nrep` back-to-back calls, HIP events
+        # on the stream around its replays (no host time in between)
+        g = hip.capture(lambda: [loop() for _ in range(nrep)], fuse=False)
+        g.replay()
+        torch.cuda.synchronize()
+        a, b = mkev(), mkev()
+        reps = max(3, args.steps // 5)
+        nat.lib.op2hip_event_record(a, ctypes.c_void_p(s))
+        for _ in range(reps):
+            g.replay()
+        nat.lib.op2hip_event_record(b, ctypes.c_void_p(s))
+        torch.cuda.synchronize()
+        ms = ctypes.c_float()
+        nat.lib.op2hip_event_elapsed_ms(a, b, ctypes.byref(ms))
+        return ms.value / 1e3 / (reps * nrep)
+
+    def cold_avg(loop, flush, nrep=40):
+        # one loop after a write and then a read of `flush` (larger than the
+        # L2s and the 256 MB Infinity Cache together), so its inputs come
+        # from HBM; the read leaves the caches holding clean lines, so the
+        # loop does not pay for writing back the flush's own data
+        g = hip.capture(loop, fuse=False)
+        g.replay()
+        torch.cuda.synchronize()
+        evs = [(mkev(), mkev()) for _ in range(nrep)]
+        for i, (a, b) in enumerate(evs):
+            flush.fill_(float(i))
+            flush.sum()
+            nat.lib.op2hip_event_record(a, ctypes.c_void_p(s))
+            g.replay()
+            nat.lib.op2hip_event_record(b, ctypes.c_void_p(s))
+        torch.cuda.synchronize()
+        tot, ms = 0.0, ctypes.c_float()
+        for a, b in evs:
+            nat.lib.op2hip_event_elapsed_ms(a, b, ctypes.byref(ms))
+            tot += ms.value
+        return tot / 1e3 / nrep
+
+    res_avg_s = hot_avg(A.res_calc)
     res_gbs = ub['res_calc'] / res_avg_s / 1e9
+    flush = torch.empty(FLUSH_BYTES // 8, dtype=torch.float64, device='cuda')
+    cold = {'flush_bytes': FLUSH_BYTES}
+    for name, loop in (('res_calc', A.res_calc), ('adt_calc', A.adt_calc)):
+        hot_s = res_avg_s if name == 'res_calc' else hot_avg(loop)
+        cold_s = cold_avg(loop, flush)
+        cold[name] = {'unique_bytes': int(ub[name]), 'hot_us': round(hot_s * 1e6, 2),
+                      'hot_frac': round(ub[name] / hot_s / 1e9 / HBM_PEAK_GBS, 4),
+                      'cold_us': round(cold_s * 1e6, 2),
+                      'cold_frac': round(ub[name] / cold_s / 1e9 / HBM_PEAK_GBS, 4)}
+    del flush
 
     value = step_bytes * args.steps / elapsed / 1e9
     out = {
@@ -211,6 +258,9 @@ def main():
                      'frac': round(res_gbs / HBM_PEAK_GBS, 4),
                      'avg_us': round(res_avg_s * 1e6, 2),
                      'bytes_per_launch': int(ub['res_calc']), 'traffic': None},
+        # res_calc and adt_calc alone, back to back (hot) and each after a
+        # 512 MiB write and read that evict L2 and the Infinity Cache (cold)
+        'cache_cold': cold,
     }
     if args.cpu_seconds > 0:
         out['cpu_baseline'] = cpu_baseline(mesh, args.cpu_seconds, step_bytes)

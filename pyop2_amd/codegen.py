@@ -152,8 +152,11 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                  stage_reads=True, stage_writes=True, unroll=1, owner_stage=()):
         assert mode in ('stream', 'coloured', 'owner')
         # owner write-back: 1 issues the loads locating a thread's first
-        # target before the elements run (they overlap), 0 after them
+        # target before the elements run (they overlap), 0 after them; 2
+        # also loads that target's old value early
         self.prefetch = prefetch
+        # occupancy request (waves per SIMD) for the register budget, 0: none
+        self.min_waves = 0
         self.stage_reads = stage_reads
         self.stage_writes = stage_writes
         # elements per thread and chunk of a direct loop (their loads overlap)
@@ -639,6 +642,12 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   // clamped to the padding entry the device lists carry
   const int jf = (int)threadIdx.x < ng ? (int)threadIdx.x : ng;
   const int f_t = p_tgt[g0 + jf], f_cb = p_tgcb[g0 + jf], f_ce = p_tgcb[g0 + jf + 1];''')
+        if pf >= 2:
+            out.append('''  // and its old value (row 0 stands in when the thread owns none)
+  const int f_row = dense ? (jf < ng ? t0 + jf : 0) : f_t;
+  %(t)s f_old[%(cdim)d];
+#pragma unroll
+  for (int k = 0; k < %(cdim)d; ++k) f_old[k] = d%(pos)d[(op2_i64)f_row * %(cdim)d + k];''' % sub)
         out += ['''  for (int l = threadIdx.x; l < nex; l += %d) {
     const unsigned code = l < rn ? (unsigned)(rlo + l) : (unsigned)p_exec[ex0 + l - rn];
     const int e = (int)(code & 0x7fffffffu);
@@ -668,10 +677,16 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
     w_nc = p_tgcb[g0 + j + 1] - w_cb;''')
         out.append('    int w_code[%(cmax)d];' % sub)
         out.append(codes)
-        out.append('''    %(t)s acc[%(cdim)d];
+        if pf >= 2:
+            out.append('''    %(t)s acc[%(cdim)d];
 #pragma unroll
-    for (int k = 0; k < %(cdim)d; ++k) acc[k] = d%(pos)d[(op2_i64)w_t * %(cdim)d + k];
+    for (int k = 0; k < %(cdim)d; ++k)
+      acc[k] = j == (int)threadIdx.x ? f_old[k] : d%(pos)d[(op2_i64)w_t * %(cdim)d + k];''' % sub)
+        else:
+            out.append('''    %(t)s acc[%(cdim)d];
 #pragma unroll
+    for (int k = 0; k < %(cdim)d; ++k) acc[k] = d%(pos)d[(op2_i64)w_t * %(cdim)d + k];''' % sub)
+        out.append('''#pragma unroll
     for (int u = 0; u < %(cmax)d; ++u) {
       if (u < w_nc) {
         const int slot = w_code[u] / %(nac)d, ac = w_code[u] - slot * %(nac)d;
@@ -879,8 +894,9 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                'inline',
                self.kernel.code,
                '#pragma clang force_cuda_host_device end', '']
-        out.append('extern "C" __global__ void __launch_bounds__(%d) %s(\n    %s) {'
-                   % (bs, self.entry, ',\n    '.join(self._params_decl())))
+        bounds = '%d, %d' % (bs, self.min_waves) if self.min_waves else '%d' % bs
+        out.append('extern "C" __global__ void __launch_bounds__(%s) %s(\n    %s) {'
+                   % (bounds, self.entry, ',\n    '.join(self._params_decl())))
         decls, compute = self._element_body()
         if self.mode == 'stream':
             out += self._stream_body(decls, compute)

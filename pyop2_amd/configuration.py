@@ -35,7 +35,9 @@ DEFAULTS = {
     'hip_owner_stage': False,     # owner blocks stage their owned rows of gathered Dats in LDS
     'hip_chain': True,            # fuse consecutive owner loops into one Dat in fusing captures
     'hip_chain_waves': 8,         # waves per SIMD a chain kernel's registers are budgeted for
-    'hip_owner_prefetch': 1,      # owner write-back lookups issued before the elements run (0/1)
+    'hip_owner_prefetch': 1,      # owner write-back loads issued before the elements run:
+                                  # 0 none, 1 the first target's lookups, 2 also its old value
+    'hip_owner_waves': 0,         # waves per SIMD owner loops are compiled for (0: compiler's choice)
     'hip_layout': True,           # locality renumbering of device storage
     'hip_layout_order': 'bisect',  # adjacency order of sets no map orders: 'bisect' or 'bfs'
     'hip_cache_dir': None,        # on-disk code-object cache (None: ~/.cache/pyop2_amd)

@@ -1055,12 +1055,15 @@ class ParLoop(base.ParLoop):
         sw = bool(cfg.get('hip_stage_writes', True))
         un = int(cfg.get('hip_stream_unroll', 1))
         st = self._owner_stage() if mode == 'owner' and cfg.get('hip_owner_stage') else ()
-        key = key + (mode, bs, pf, sr, sw, un, st)
+        mat = any(a._is_mat for a in self.args)
+        mw = int(cfg.get('hip_owner_waves', 0)) if mode == 'owner' and not mat else 0
+        key = key + (mode, bs, pf, sr, sw, un, st, mw)
         c = _loop_cache.get(key)
         if c is None:
             src = LoopSource(self.kernel, self._it_space.extents, self.args, consts, mode, bs,
                              prefetch=pf, stage_reads=sr, stage_writes=sw, unroll=un,
                              owner_stage=st)
+            src.min_waves = mw
             c = _CompiledLoop(src)
             _loop_cache[key] = c
         return c
@@ -1427,7 +1430,7 @@ _VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill', 'hi
                  'hip_owner_mat_block_size', 'hip_owner_mat_fill',
                  'hip_owner_targets', 'hip_owner', 'hip_partition_size', 'hip_reorder',
                  'hip_stage_reads', 'hip_stage_writes', 'hip_stream_unroll',
-                 'hip_stream_blocks', 'hip_owner_prefetch')
+                 'hip_stream_blocks', 'hip_owner_prefetch', 'hip_owner_waves')
 
 
 def _written_buffers(pl):

@@ -99,10 +99,12 @@ def test_owner_mode_bit_exact(pair):
 
 @pytest.mark.parametrize('opts', [{'hip_owner_stage': True},
                                   {'hip_owner_block_size': 256, 'hip_owner_fill': 0.6},
-                                  {'hip_owner_prefetch': 0}])
+                                  {'hip_owner_prefetch': 0},
+                                  {'hip_owner_prefetch': 2, 'hip_owner_waves': 8}])
 def test_owner_variants_bit_exact(pair, opts):
     """Launch variants of the owner template (owned rows of q and adt staged
-    in LDS, other block sizes and owned ranges, no prefetch) keep the
+    in LDS, other block sizes and owned ranges, no prefetch, old values
+    loaded early under an 8-wave register budget) keep the
     element-ordered sums: res and q stay bit-exact to the oracle."""
     from pyop2_amd import configuration as cfg
     A, B = pair
