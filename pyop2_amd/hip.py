@@ -227,14 +227,23 @@ class Sparsity(base.Sparsity):
     _cache = {}
 
     def _build(self):
-        rmult, cmult = self._dims
-        nrows = self._nrows
-        lsize = nrows * rmult
-        s = stream()
         for m in self._rmaps + self._cmaps:
             v = m.values
             if v.size and (v.min() < 0 or v.max() >= m.dataset.total_size):
                 raise base.MapValueError("Map %s has values outside its dataset" % m.name)
+        (self._dev_rowptr, self._dev_colidx, self._dev_nnz_rows,
+         self._d_nz) = self._build_device()
+        self._o_nnz = np.zeros(self._nrows * self._dims[0], dtype=np.int32)
+        self._o_nz = 0
+        self._user = None
+
+    def _build_device(self):
+        """The device CSR of the maps: (rowptr, colidx, nnz per row, nnz),
+        enqueued on the current stream."""
+        rmult, cmult = self._dims
+        nrows = self._nrows
+        lsize = nrows * rmult
+        s = stream()
         rmaps = [m._device_values() for m in self._rmaps]
         cmaps = [m._device_values() for m in self._cmaps]
         rowptr = torch.empty(lsize + 1, dtype=torch.int32, device=device())
@@ -253,13 +262,7 @@ class Sparsity(base.Sparsity):
                                                ctypes.c_void_p(_ptr(colidx)),
                                                ctypes.c_void_p(_ptr(nnz_rows)),
                                                ctypes.c_void_p(s)), 'sparsity fill')
-        self._dev_rowptr = rowptr
-        self._dev_colidx = colidx
-        self._dev_nnz_rows = nnz_rows
-        self._d_nz = int(nnz.value)
-        self._o_nnz = np.zeros(lsize, dtype=np.int32)
-        self._o_nz = 0
-        self._user = None
+        return rowptr, colidx, nnz_rows, int(nnz.value)
 
     def _row_layout(self):
         return layout.of(self._rmaps[0].dataset)
