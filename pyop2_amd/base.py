@@ -400,8 +400,11 @@ class Dat(DataCarrier):
     @validate_type(('dataset', Set, SetTypeError), ('name', str, NameTypeError))
     def __init__(self, dataset, data=None, dtype=None, name=None, soa=None, uid=None):
         if data is None:
-            data = np.zeros(dataset.total_size * dataset.cdim,
-                            dtype=dtype if dtype is not None else np.float64)
+            try:
+                dt = np.dtype(dtype if dtype is not None else np.float64)
+            except TypeError:
+                raise DataTypeError("Invalid data type: %s" % dtype)
+            data = np.zeros(dataset.total_size * dataset.cdim, dtype=dt)
         self._dataset = dataset
         self._data = verify_reshape(data, dtype, (dataset.total_size,) + dataset.dim,
                                     allow_none=True)
@@ -709,8 +712,13 @@ class Map(object):
         self._uid = Map._globalcount
         Map._globalcount += 1
 
-    @validate_type(('index', (int, IterationIndex), IndexTypeError))
     def __getitem__(self, index):
+        if isinstance(index, slice):
+            raise NotImplementedError("Slicing maps is not allowed")
+        return self._index(index)
+
+    @validate_type(('index', (int, IterationIndex), IndexTypeError))
+    def _index(self, index):
         if isinstance(index, int) and not (0 <= index < self._dim):
             raise IndexValueError("Index must be in interval [0,%d]" % (self._dim - 1))
         if isinstance(index, IterationIndex) and index.index not in [0, 1]:

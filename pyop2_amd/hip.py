@@ -770,6 +770,8 @@ class _CompiledLoop(object):
 
 
 _loop_cache = {}
+# generated sources not compiled yet, by the same keys
+_source_cache = {}
 
 
 class JITModule(base.JITModule):
@@ -801,8 +803,10 @@ class _LaunchEntry(object):
     def __init__(self, pl, mode, bs):
         consts = base.Const._definitions()
         self.consts = consts
-        loop = pl._compiled(mode, bs, consts)
-        self.loop = loop
+        # the generated source decides the owner plan; it is compiled once
+        # the plan exists (a loop falling back to colouring compiles only
+        # its coloured form)
+        _, src = pl._source(mode, bs, consts)
         args = pl.args
         n = pl._it_space.exec_size if pl.needs_exec_halo else pl._it_space.size
         iterset = pl._it_space.iterset
@@ -825,8 +829,7 @@ class _LaunchEntry(object):
         plan = None
         lds = 0
         if mode == 'owner':
-            slot_bytes = loop.source.owner_slot_bytes()
-            src = loop.source
+            slot_bytes = src.owner_slot_bytes()
             plan = get_owner_plan(iterset, n, args, slot_bytes, block=bs,
                                   lds_of=lambda p: src.owner_lds_bytes(p.k, p.max_exec, p.max_codes))
             if plan is None:
@@ -840,10 +843,12 @@ class _LaunchEntry(object):
             nparts = plan.nblocks
             grid_stream = 0
         else:
-            chunk = bs * loop.source.stream_unroll()
+            chunk = bs * src.stream_unroll()
             grid_stream = max(1, min((n + chunk - 1) // chunk,
                                      int(cfg.get('hip_stream_blocks') or 4096)))
             nparts = grid_stream
+        loop = pl._compiled(mode, bs, consts)
+        self.loop = loop
         partials = {}
         vals = []
         for role, pos in loop.source.param_roles():
@@ -1052,6 +1057,17 @@ class ParLoop(base.ParLoop):
         return tuple(out)
 
     def _compiled(self, mode, bs, consts):
+        """The compiled loop of this loop's signature for ``mode`` (cached)."""
+        key, src = self._source(mode, bs, consts)
+        c = _loop_cache.get(key)
+        if c is None:
+            c = _CompiledLoop(src)
+            _loop_cache[key] = c
+        return c
+
+    def _source(self, mode, bs, consts):
+        """(cache key, generated source) of this loop's signature for
+        ``mode`` under the current configuration."""
         key = base.JITModule._cache_key(self.kernel, self._it_space.extents, *self.args)
         pf = int(cfg.get('hip_owner_prefetch', 1))
         sr = bool(cfg.get('hip_stage_reads', False))
@@ -1062,14 +1078,18 @@ class ParLoop(base.ParLoop):
         mw = int(cfg.get('hip_owner_waves', 0)) if mode == 'owner' and not mat else 0
         key = key + (mode, bs, pf, sr, sw, un, st, mw)
         c = _loop_cache.get(key)
-        if c is None:
+        if c is not None:
+            return key, c.source
+        src = _source_cache.get(key)
+        if src is None:
             src = LoopSource(self.kernel, self._it_space.extents, self.args, consts, mode, bs,
                              prefetch=pf, stage_reads=sr, stage_writes=sw, unroll=un,
                              owner_stage=st)
             src.min_waves = mw
-            c = _CompiledLoop(src)
-            _loop_cache[key] = c
-        return c
+            if len(_source_cache) >= _CALL_CACHE_MAX:
+                _source_cache.pop(next(iter(_source_cache)))
+            _source_cache[key] = src
+        return key, src
 
     def _entry(self, mode):
         """A launch entry for ``mode`` under the current configuration
