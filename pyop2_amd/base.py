@@ -509,12 +509,22 @@ class Dat(DataCarrier):
         return _make_object('Dat', self.dataset,
                             op(self.data_ro, as_type(other.data_ro, self.dtype)), self.dtype)
 
+    _BINARY = {operator.iadd: operator.add, operator.isub: operator.sub,
+               operator.imul: operator.mul, operator.itruediv: operator.truediv}
+
     def _iop(self, other, op):
         if np.isscalar(other):
-            op(self.data, as_type(other, self.dtype))
+            val = as_type(other, self.dtype)
         else:
             self._check_shape(other)
-            op(self.data, as_type(other.data_ro, self.dtype))
+            val = as_type(other.data_ro, self.dtype)
+        tgt = self.data
+        res = self._BINARY.get(op, op)(tgt, val)
+        if res.shape != tgt.shape:
+            raise ValueError("operands of shapes %s and %s do not combine in place"
+                             % (tgt.shape, np.shape(val)))
+        # the Dat keeps its dtype (integer division truncates)
+        np.copyto(tgt, res, casting='unsafe')
         return self
 
     def __add__(self, other):
