@@ -58,6 +58,37 @@ void rhs_vec_tri(double **A, double *x[2], double **f)
 }
 """
 
+RHS_TRI_ELEM = """
+void rhs_tri_elem(double **A, double *x[2], double **f)
+{
+  const double phi[3][3] = {{2.0/3.0, 1.0/6.0, 1.0/6.0},
+                            {1.0/6.0, 2.0/3.0, 1.0/6.0},
+                            {1.0/6.0, 1.0/6.0, 2.0/3.0}};
+  double det = (x[1][0] - x[0][0]) * (x[2][1] - x[0][1])
+             - (x[2][0] - x[0][0]) * (x[1][1] - x[0][1]);
+  double w = fabs(det) / 6.0;
+  for (int j = 0; j < 3; ++j)
+    for (int q = 0; q < 3; ++q) {
+      double fq = phi[q][0] * f[0][0] + phi[q][1] * f[1][0] + phi[q][2] * f[2][0];
+      A[j][0] += phi[q][j] * fq * w;
+    }
+}
+"""
+
+RHS_VEC_TRI_ITSPACE = """
+void rhs_vec_tri_its(double *A, double *x[2], double **f, int j)
+{
+  double det = (x[1][0] - x[0][0]) * (x[2][1] - x[0][1])
+             - (x[2][0] - x[0][0]) * (x[1][1] - x[0][1]);
+  double a = fabs(det) / 24.0;
+  for (int r = 0; r < 2; ++r)
+    A[r] += a * (2.0 * f[j][r] + f[(j + 1) % 3][r] + f[(j + 2) % 3][r]);
+}
+"""
+
+ZERO_DAT = "void zero_dat(double *d) { *d = 0.0; }"
+ZERO_VEC_DAT = "void zero_vec_dat(double *d) { d[0] = 0.0; d[1] = 0.0; }"
+
 
 @pytest.fixture
 def mesh(backend):
@@ -222,6 +253,40 @@ class TestMatrices:
         x = op2.Dat(mesh['vnodes'], np.zeros((4, 2)), np.float64, "x")
         op2.solve(vm, x, b)
         assert_allclose(x.data, mesh['f_vec'].data, 1e-12)
+
+    def test_zero_rhs(self, mesh):
+        """A WRITE loop zeroes the right-hand side (reference
+        test_matrices.py:693-697)."""
+        b = rhs(mesh)
+        op2.par_loop(op2.Kernel(ZERO_DAT, "zero_dat"), mesh['nodes'],
+                     b(op2.IdentityMap, op2.WRITE))
+        assert all(b.data_ro.ravel() == 0.0)
+
+    def test_rhs_whole_element(self, mesh):
+        """The right-hand side from a kernel that fills all three rows of an
+        element through a vector map (reference rhs_ffc, :718-726)."""
+        b = op2.Dat(mesh['nodes'], np.zeros(4), np.float64, "b")
+        op2.par_loop(op2.Kernel(RHS_TRI_ELEM, "rhs_tri_elem"), mesh['elements'],
+                     b(mesh['elem_node'], op2.INC), mesh['coords'](mesh['elem_vnode'], op2.READ),
+                     mesh['f'](mesh['elem_node'], op2.READ))
+        assert_allclose(b.data, EXPECTED_RHS, 1e-6)
+
+    def test_rhs_vector_itspace(self, mesh):
+        """The vector right-hand side one row per iteration-space index,
+        after zeroing it with a WRITE loop (reference :751-763)."""
+        b = op2.Dat(mesh['vnodes'], np.ones((4, 2)), np.float64, "b")
+        op2.par_loop(op2.Kernel(ZERO_VEC_DAT, "zero_vec_dat"), mesh['vnodes'],
+                     b(op2.IdentityMap, op2.WRITE))
+        op2.par_loop(op2.Kernel(RHS_VEC_TRI_ITSPACE, "rhs_vec_tri_its"), mesh['elements'](3),
+                     b(mesh['elem_vnode'][op2.i[0]], op2.INC),
+                     mesh['coords'](mesh['elem_vnode'], op2.READ),
+                     mesh['f_vec'](mesh['elem_vnode'], op2.READ))
+        assert_allclose(b.data, EXPECTED_VEC_RHS, 1e-6)
+
+    def test_zero_vector_matrix(self, mesh):
+        vm = vec_mass_mat(mesh)
+        vm.zero()
+        assert_allclose(vm.values, np.zeros((8, 8)), 1e-14)
 
     def test_solver_nonconvergence_raises(self, mesh):
         mat = mass_mat(mesh)
