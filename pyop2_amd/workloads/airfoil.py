@@ -14,6 +14,7 @@ backend module) and returns an :class:`Airfoil` whose ``iteration()`` runs
 one time step.
 """
 
+import re
 from math import atan, sqrt
 
 import numpy as np
@@ -177,6 +178,33 @@ def unique_bytes(mesh):
     }
 
 
+def _vector_form(code, signature, names):
+    """A kernel string with its pointer arguments ``names`` (old -> new
+    element expression prefix) rewritten for vector maps: ``x1[k]`` becomes
+    ``x[0][k]`` and so on; the arithmetic is untouched."""
+    head, body = code.split('{', 1)
+    for old, new in names:
+        body = re.sub(r'\(\*%s\)' % old, '(%s[0])' % new, body)
+        body = re.sub(r'\b%s\[' % old, '%s[' % new, body)
+    return signature + ' {' + body
+
+
+# The vector-map form of adt_calc, res_calc and bres_calc (the reference's
+# demo/airfoil_vector.py drives them, demo/airfoil_vector_kernels.py holds
+# them): the same arithmetic as the kernels above, reached through pointer
+# arrays of vector maps.
+ADT_CALC_VEC = _vector_form(
+    ADT_CALC, '\nvoid adt_calc(double *x[2], double *q, double *adt)',
+    [('x1', 'x[0]'), ('x2', 'x[1]'), ('x3', 'x[2]'), ('x4', 'x[3]')])
+RES_CALC_VEC = _vector_form(
+    RES_CALC, '\nvoid res_calc(double *x[2], double *q[4], double *adt[1], double *res[4])',
+    [('x1', 'x[0]'), ('x2', 'x[1]'), ('q1', 'q[0]'), ('q2', 'q[1]'), ('adt1', 'adt[0]'),
+     ('adt2', 'adt[1]'), ('res1', 'res[0]'), ('res2', 'res[1]')])
+BRES_CALC_VEC = _vector_form(
+    BRES_CALC, '\nvoid bres_calc(double *x[2], double *q1, double *adt1, double *res1, int *bound)',
+    [('x1', 'x[0]'), ('x2', 'x[1]')])
+
+
 class Airfoil(object):
     """Sets, maps, dats and kernels of one airfoil problem in world ``W``."""
 
@@ -261,3 +289,34 @@ class Airfoil(object):
             self.res_calc()
             self.bres_calc()
             self.update()
+
+
+class AirfoilVector(Airfoil):
+    """The same problem driven as the reference's vector-map demo
+    (demo/airfoil_vector.py:84-135): adt_calc reads the four nodes of a
+    cell through one vector map, res_calc reads and increments through the
+    edge's vector maps, bres_calc reads its nodes through one.  Results
+    equal :class:`Airfoil`'s bit for bit."""
+
+    def __init__(self, W, mesh, q0=None, consts=True):
+        super(AirfoilVector, self).__init__(W, mesh, q0, consts)
+        self.k_adt_calc = W.Kernel(ADT_CALC_VEC, 'adt_calc')
+        self.k_res_calc = W.Kernel(RES_CALC_VEC, 'res_calc')
+        self.k_bres_calc = W.Kernel(BRES_CALC_VEC, 'bres_calc')
+
+    def adt_calc(self):
+        W = self.W
+        W.par_loop(self.k_adt_calc, self.cells, self.p_x(self.pcell, W.READ),
+                   self.p_q(W.IdentityMap, W.READ), self.p_adt(W.IdentityMap, W.WRITE))
+
+    def res_calc(self):
+        W = self.W
+        W.par_loop(self.k_res_calc, self.edges, self.p_x(self.pedge, W.READ),
+                   self.p_q(self.pevcell, W.READ), self.p_adt(self.pecell, W.READ),
+                   self.p_res(self.pevcell, W.INC))
+
+    def bres_calc(self):
+        W = self.W
+        W.par_loop(self.k_bres_calc, self.bedges, self.p_x(self.pbedge, W.READ),
+                   self.p_q(self.pbevcell[0], W.READ), self.p_adt(self.pbecell[0], W.READ),
+                   self.p_res(self.pbevcell[0], W.INC), self.p_bound(W.IdentityMap, W.READ))
