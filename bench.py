@@ -40,6 +40,8 @@ def parse():
     p.add_argument('--steps', type=int, default=1000)
     p.add_argument('--warmup', type=int, default=20)
     p.add_argument('--nx', type=int, default=1200)
+    p.add_argument('--vector-maps', type=int, default=0,
+                   help='drive the step in the vector-map form (1) instead of the scalar one')
     p.add_argument('--ny', type=int, default=600)
     p.add_argument('--graph', type=int, default=1, help='replay each step as a hipGraph')
     p.add_argument('--fuse', type=int, default=0,
@@ -138,7 +140,7 @@ def main():
         opts['hip_owner_targets'] = args.owner_targets
     op2.init(**opts)
     mesh = meshes.QuadMesh(args.nx, args.ny, seed=0)
-    A = airfoil.Airfoil(op2, mesh)
+    A = (airfoil.AirfoilVector if args.vector_maps else airfoil.Airfoil)(op2, mesh)
     ub = airfoil.unique_bytes(mesh)
     step_bytes = ub['save_soln'] + 2 * (ub['adt_calc'] + ub['res_calc'] + ub['bres_calc'] +
                                         ub['update'])
@@ -251,6 +253,7 @@ def main():
                    'pct_of_hbm_peak': round(100.0 * value / HBM_PEAK_GBS, 2),
                    'graph': bool(args.graph),
                    'fused_loops': bool(args.graph and args.fuse),
+                   'vector_maps': bool(args.vector_maps),
                    'autotune': _tuned() if args.autotune else False,
                    'partition_size': hip.cfg.get('hip_partition_size') or 'auto'},
         'roofline': {'bound': 'hbm', 'kernel': 'op2_res_calc_owner (one res_calc par_loop)',

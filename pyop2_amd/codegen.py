@@ -157,6 +157,10 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         self.prefetch = prefetch
         # occupancy request (waves per SIMD) for the register budget, 0: none
         self.min_waves = 0
+        # argument position -> position whose map parameter it reads (args
+        # whose maps share one device array read one parameter, so the
+        # element's map entries are loaded once)
+        self.map_of = {}
         self.stage_reads = stage_reads
         self.stage_writes = stage_writes
         # elements per thread and chunk of a direct loop (their loads overlap)
@@ -332,18 +336,18 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                     loads.append('{ const int tt = m%d[%s * %d + %d]; const %s* src = '
                                  '(unsigned)(tt - t0) < (unsigned)nn_own ? s_st%d + (tt - t0) * %d : '
                                  'd%d + (op2_i64)tt * %d; for (int k = 0; k < %d; ++k) %s[k] = src[k]; }'
-                                 % (p, E, a.mdim, a.idx, t, self._stage_of[p], a.cdim, p, a.cdim,
-                                    a.cdim, n('rl')))
+                                 % (self._m(p), E, a.mdim, a.idx, t, self._stage_of[p], a.cdim, p,
+                                    a.cdim, a.cdim, n('rl')))
                     call_args.append(n('rl'))
                 elif local:
                     decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
                     loads.append('{ const %s* src = d%d + (op2_i64)m%d[%s * %d + %d] * %d; '
                                  'for (int k = 0; k < %d; ++k) %s[k] = src[k]; }'
-                                 % (t, p, p, E, a.mdim, a.idx, a.cdim, a.cdim, n('rl')))
+                                 % (t, p, self._m(p), E, a.mdim, a.idx, a.cdim, a.cdim, n('rl')))
                     call_args.append(n('rl'))
                 else:
                     call_args.append('d%d + (op2_i64)m%d[%s * %d + %d] * %d'
-                                     % (p, p, E, a.mdim, a.idx, a.cdim))
+                                     % (p, self._m(p), E, a.mdim, a.idx, a.cdim))
             elif a.kind == 'vec':
                 decls.append('%s* %s[%d];' % (t, n('v'), a.mdim))
                 if a.race:
@@ -356,12 +360,12 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                     loads.append('for (int j = 0; j < %d; ++j) { const %s* src = d%d + '
                                  '(op2_i64)m%d[%s * %d + j] * %d; %s[j] = %s[j]; '
                                  'for (int k = 0; k < %d; ++k) %s[j][k] = src[k]; }'
-                                 % (a.mdim, t, p, p, E, a.mdim, a.cdim, n('v'), n('rl'),
+                                 % (a.mdim, t, p, self._m(p), E, a.mdim, a.cdim, n('v'), n('rl'),
                                     a.cdim, n('rl')))
                 else:
                     compute.append('for (int j = 0; j < %d; ++j) %s[j] = d%d + '
                                    '(op2_i64)m%d[%s * %d + j] * %d;'
-                                   % (a.mdim, n('v'), p, p, E, a.mdim, a.cdim))
+                                   % (a.mdim, n('v'), p, self._m(p), E, a.mdim, a.cdim))
                 call_args.append(n('v'))
             elif a.kind == 'itspace':
                 ivar = 'i_%d' % a.iidx
@@ -372,7 +376,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                     call_args.append('%s[%s]' % (n('inc'), ivar))
                 else:
                     call_args.append('d%d + (op2_i64)m%d[%s * %d + %s] * %d'
-                                     % (p, p, E, a.mdim, ivar, a.cdim))
+                                     % (p, self._m(p), E, a.mdim, ivar, a.cdim))
             elif a.kind == 'mat':
                 e0 = ext[0] if len(ext) > 0 else 1
                 e1 = ext[1] if len(ext) > 1 else 1
@@ -407,6 +411,10 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         if split:
             return decls, loads, compute
         return decls, loads + compute
+
+    def _m(self, p):
+        """Position of the map parameter argument ``p`` reads."""
+        return self.map_of.get(p, p)
 
     def _apply_increments(self):
         """Apply private INC increments and Mat entries of element ``e``."""

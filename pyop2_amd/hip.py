@@ -803,10 +803,6 @@ class _LaunchEntry(object):
     def __init__(self, pl, mode, bs):
         consts = base.Const._definitions()
         self.consts = consts
-        # the generated source decides the owner plan; it is compiled once
-        # the plan exists (a loop falling back to colouring compiles only
-        # its coloured form)
-        _, src = pl._source(mode, bs, consts)
         args = pl.args
         n = pl._it_space.exec_size if pl.needs_exec_halo else pl._it_space.size
         iterset = pl._it_space.iterset
@@ -826,6 +822,11 @@ class _LaunchEntry(object):
                 self.to_device.append(a.data)
         for d in self.to_device:
             d._to_device()
+        # the generated source (layouts decided: it names which arguments
+        # share a map array) decides the owner plan; it is compiled once the
+        # plan exists (a loop falling back to colouring compiles only its
+        # coloured form)
+        _, src = pl._source(mode, bs, consts)
         plan = None
         lds = 0
         if mode == 'owner':
@@ -1076,7 +1077,8 @@ class ParLoop(base.ParLoop):
         st = self._owner_stage() if mode == 'owner' and cfg.get('hip_owner_stage') else ()
         mat = any(a._is_mat for a in self.args)
         mw = int(cfg.get('hip_owner_waves', 0)) if mode == 'owner' and not mat else 0
-        key = key + (mode, bs, pf, sr, sw, un, st, mw)
+        alias = self._map_alias()
+        key = key + (mode, bs, pf, sr, sw, un, st, mw, alias)
         c = _loop_cache.get(key)
         if c is not None:
             return key, c.source
@@ -1086,10 +1088,25 @@ class ParLoop(base.ParLoop):
                              prefetch=pf, stage_reads=sr, stage_writes=sw, unroll=un,
                              owner_stage=st)
             src.min_waves = mw
+            src.map_of = {i: r for i, r in enumerate(alias) if r != i}
             if len(_source_cache) >= _CALL_CACHE_MAX:
                 _source_cache.pop(next(iter(_source_cache)))
             _source_cache[key] = src
         return key, src
+
+    def _map_alias(self):
+        """Per argument, the first argument whose map has the same device
+        array (and arity); Dat arguments through maps that share one array
+        (the same Map, or identical maps whose target Sets share a layout)
+        then read one map parameter, so the kernel loads each map entry once.
+        Needs the layouts decided: it asks for the maps' device arrays."""
+        seen, out = {}, []
+        for i, a in enumerate(self.args):
+            r = i
+            if a._is_indirect:
+                r = seen.setdefault((a.map._device_values().data_ptr(), a.map.dim), i)
+            out.append(r)
+        return tuple(out)
 
     def _entry(self, mode):
         """A launch entry for ``mode`` under the current configuration
