@@ -48,6 +48,8 @@ def parse():
                    help='fuse consecutive owner loops into one Dat in the captured step')
     p.add_argument('--chain-waves', type=int, default=-1,
                    help='waves per SIMD a fused kernel is budgeted for (-1: default, 0: none)')
+    p.add_argument('--cold', type=int, default=1,
+                   help='time res_calc and adt_calc cache-cold too (0: skip)')
     p.add_argument('--cpu-seconds', type=float, default=15.0,
                    help='approximate CPU time budget of the oracle baseline (0: skip)')
     p.add_argument('--partition-size', type=int, default=0)
@@ -217,16 +219,18 @@ def main():
 
     res_avg_s = hot_avg(A.res_calc)
     res_gbs = ub['res_calc'] / res_avg_s / 1e9
-    flush = torch.empty(FLUSH_BYTES // 8, dtype=torch.float64, device='cuda')
-    cold = {'flush_bytes': FLUSH_BYTES}
-    for name, loop in (('res_calc', A.res_calc), ('adt_calc', A.adt_calc)):
-        hot_s = res_avg_s if name == 'res_calc' else hot_avg(loop)
-        cold_s = cold_avg(loop, flush)
-        cold[name] = {'unique_bytes': int(ub[name]), 'hot_us': round(hot_s * 1e6, 2),
-                      'hot_frac': round(ub[name] / hot_s / 1e9 / HBM_PEAK_GBS, 4),
-                      'cold_us': round(cold_s * 1e6, 2),
-                      'cold_frac': round(ub[name] / cold_s / 1e9 / HBM_PEAK_GBS, 4)}
-    del flush
+    cold = None
+    if args.cold:
+        flush = torch.empty(FLUSH_BYTES // 8, dtype=torch.float64, device='cuda')
+        cold = {'flush_bytes': FLUSH_BYTES}
+        for name, loop in (('res_calc', A.res_calc), ('adt_calc', A.adt_calc)):
+            hot_s = res_avg_s if name == 'res_calc' else hot_avg(loop)
+            cold_s = cold_avg(loop, flush)
+            cold[name] = {'unique_bytes': int(ub[name]), 'hot_us': round(hot_s * 1e6, 2),
+                          'hot_frac': round(ub[name] / hot_s / 1e9 / HBM_PEAK_GBS, 4),
+                          'cold_us': round(cold_s * 1e6, 2),
+                          'cold_frac': round(ub[name] / cold_s / 1e9 / HBM_PEAK_GBS, 4)}
+        del flush
 
     value = step_bytes * args.steps / elapsed / 1e9
     out = {
