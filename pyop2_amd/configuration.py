@@ -37,6 +37,8 @@ DEFAULTS = {
     'hip_chain_waves': 8,         # waves per SIMD a chain kernel's registers are budgeted for
     'hip_owner_prefetch': 1,      # owner write-back loads issued before the elements run:
                                   # 0 none, 1 the first target's lookups, 2 also its old value
+    'hip_owner_mat_lds': 0,       # LDS bytes an owner Mat block may use (0: a CU's share at
+                                  # full occupancy, 20 KB for 256 threads)
     'hip_owner_waves': 0,         # waves per SIMD owner loops are compiled for (0: compiler's choice)
     'hip_layout': True,           # locality renumbering of device storage
     'hip_layout_order': 'bisect',  # adjacency order of sets no map orders: 'bisect' or 'bfs'

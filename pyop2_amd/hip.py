@@ -645,6 +645,9 @@ class OwnerPlan(object):
 
 _owner_cache = {}
 OWNER_LDS_LIMIT = 64 * 1024
+# MI355X (gfx950) compute unit: 160 KB of LDS, 4 SIMDs of at most 8 waves
+LDS_PER_CU = 160 * 1024
+WAVES_PER_CU = 32
 
 
 def mat_program_host(plan, cmap, e1, nnodes, rowptr, colidx, rd, cd):
@@ -704,17 +707,19 @@ def _owner_columns(args):
     return cols
 
 
-def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None):
+def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None,
+                   lds_limit=None):
     """Cached owner plan.  The owned range ``k`` is chosen so a block runs
     about one element per thread (a first build measures elements run per
     owned target), and shrunk until a block's LDS (``lds_of(plan)``; by
-    default ``slot_bytes`` per element) fits ``OWNER_LDS_LIMIT`` bytes."""
+    default ``slot_bytes`` per element) fits ``lds_limit`` (default
+    ``OWNER_LDS_LIMIT``) bytes."""
     if lds_of is None:
         lds_of = lambda p: p.max_exec * slot_bytes
     cols = _owner_columns(args)
     target = cols[0][0].dataset
     k_cfg = k or cfg.get('hip_owner_targets')
-    key = (id(iterset), n, k_cfg, block, cfg.get('hip_owner_fill'),
+    key = (id(iterset), n, k_cfg, block, cfg.get('hip_owner_fill'), lds_limit,
            tuple((id(m), i) for m, i in cols))
     if key in _owner_cache:
         return _owner_cache[key]
@@ -728,6 +733,8 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None)
         return OwnerPlan(n, target.total_size, vals, kk, order)
 
     limit = cfg.get('hip_owner_lds') or min(OWNER_LDS_LIMIT, _state.get('lds_per_block') or 65536)
+    if lds_limit:
+        limit = min(limit, lds_limit)
     if k_cfg:
         kk = int(k_cfg)
         p = build(kk)
@@ -831,8 +838,17 @@ class _LaunchEntry(object):
         lds = 0
         if mode == 'owner':
             slot_bytes = src.owner_slot_bytes()
+            lds_limit = None
+            if any(a._is_mat for a in args):
+                # Mat blocks keep a local-matrix row per contribution: their
+                # LDS is kept to a CU's share per block at full occupancy (8
+                # waves per SIMD), so wide element matrices get smaller
+                # owned ranges rather than fewer blocks per CU
+                lds_limit = int(cfg.get('hip_owner_mat_lds') or
+                                LDS_PER_CU * bs // (64 * WAVES_PER_CU))
             plan = get_owner_plan(iterset, n, args, slot_bytes, block=bs,
-                                  lds_of=lambda p: src.owner_lds_bytes(p.k, p.max_exec, p.max_codes))
+                                  lds_of=lambda p: src.owner_lds_bytes(p.k, p.max_exec, p.max_codes),
+                                  lds_limit=lds_limit)
             if plan is None:
                 raise _NoOwnerPlan()
             nparts = max(plan.nactive, 1)
@@ -1470,7 +1486,8 @@ _VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill', 'hi
                  'hip_owner_mat_block_size', 'hip_owner_mat_fill',
                  'hip_owner_targets', 'hip_owner', 'hip_partition_size', 'hip_reorder',
                  'hip_stage_reads', 'hip_stage_writes', 'hip_stream_unroll',
-                 'hip_stream_blocks', 'hip_owner_prefetch', 'hip_owner_waves')
+                 'hip_stream_blocks', 'hip_owner_prefetch', 'hip_owner_waves',
+                 'hip_owner_mat_lds')
 
 
 def _written_buffers(pl):
