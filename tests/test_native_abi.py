@@ -45,3 +45,19 @@ def test_cg_system_layout_matches_library():
     from pyop2_amd import _native, cg
     import ctypes as C
     assert C.sizeof(cg.CGSystem) == _native.lib.op2hip_cgb_sizeof_system()
+
+
+def test_backend_refuses_to_run_without_the_library(tmp_path):
+    """A copy of the package without its built library cannot import the
+    hip backend: ImportError naming libop2hip.so (no silent fallback)."""
+    import shutil
+    import subprocess
+    import sys
+    pkg = tmp_path / 'pyop2_amd'
+    shutil.copytree(os.path.join(ROOT, 'pyop2_amd'), str(pkg),
+                    ignore=shutil.ignore_patterns('lib', '__pycache__', '*.so', '*.o'))
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "try:\n    import pyop2_amd.hip\nexcept ImportError as e:\n"
+            "    sys.exit(0 if 'libop2hip.so' in str(e) else 2)\nsys.exit(1)\n" % str(tmp_path))
+    r = subprocess.run([sys.executable, '-c', code], cwd=str(tmp_path))
+    assert r.returncode == 0
