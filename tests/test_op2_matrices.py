@@ -1,11 +1,15 @@
 """Sparsity, Mat assembly, zeroing and solves on every backend.
 
-Golden values are the reference's (test/unit/test_matrices.py): the CSR of
+Golden values are the reference's (test/unit/test_matrices.py, kept as data
+in tests/golden/reference_fixtures.json): the CSR of
 its 4-element fixture (:54-62), the expected P1 mass matrix, vector mass
 matrix and right-hand sides on its 2-triangle mesh (:579-608) with the
 reference's tolerances, the zero_rows results (:765-778) and x = f for
 M x = M f (:647-650, :780-783).  Kernels are the P1 kernels of
 pyop2_amd.workloads.fem (exact quadrature)."""
+
+import json
+import os
 
 import numpy as np
 import pytest
@@ -15,25 +19,16 @@ from pyop2_amd import op2
 from pyop2_amd.exceptions import MapValueError
 from pyop2_amd.workloads import fem
 
-ELEM_NODE = np.asarray([0, 1, 3, 2, 3, 1], dtype=np.uint32)
-COORDS = np.asarray([(0.0, 0.0), (2.0, 0.0), (1.0, 1.0), (0.0, 1.5)])
-EXPECTED_MATRIX = np.asarray([(0.25, 0.125, 0.0, 0.125),
-                              (0.125, 0.291667, 0.0208333, 0.145833),
-                              (0.0, 0.0208333, 0.0416667, 0.0208333),
-                              (0.125, 0.145833, 0.0208333, 0.291667)])
-EXPECTED_VECTOR_MATRIX = np.asarray([
-    (0.25, 0., 0.125, 0., 0., 0., 0.125, 0.),
-    (0., 0.25, 0., 0.125, 0., 0., 0., 0.125),
-    (0.125, 0., 0.29166667, 0., 0.02083333, 0., 0.14583333, 0.),
-    (0., 0.125, 0., 0.29166667, 0., 0.02083333, 0., 0.14583333),
-    (0., 0., 0.02083333, 0., 0.04166667, 0., 0.02083333, 0.),
-    (0., 0., 0., 0.02083333, 0., 0.04166667, 0., 0.02083333),
-    (0.125, 0., 0.14583333, 0., 0.02083333, 0., 0.29166667, 0.),
-    (0., 0.125, 0., 0.14583333, 0., 0.02083333, 0., 0.29166667)])
-EXPECTED_RHS = np.asarray([[0.9999999523522115], [1.3541666031724144],
-                           [0.2499999883507239], [1.6458332580869566]])
-EXPECTED_VEC_RHS = np.asarray([[0.5, 1.0], [0.58333333, 1.16666667],
-                               [0.08333333, 0.16666667], [0.58333333, 1.16666667]])
+_FX = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden',
+                                   'reference_fixtures.json')))
+ELEM_NODE = np.asarray(_FX['two_triangles']['elem_node'], dtype=np.uint32)
+COORDS = np.asarray(_FX['two_triangles']['coords'])
+EXPECTED_MATRIX = np.asarray(_FX['expected_matrix']['value'])
+EXPECTED_VECTOR_MATRIX = np.asarray(_FX['expected_vector_matrix']['value'])
+EXPECTED_RHS = np.asarray(_FX['expected_rhs']['value'])
+EXPECTED_VEC_RHS = np.asarray(_FX['expected_vec_rhs']['value'])
+SPARSITY_FIXTURE = _FX['sparsity_fixture']
+ZERO_ROWS = _FX['zero_rows']
 
 MASS_VEC_TRI = """
 void mass_vec_tri(double A[2][2], double *x[2], int j, int k)
@@ -136,11 +131,10 @@ class TestSparsity:
     def test_build_sparsity(self, backend):
         elements = op2.Set(4)
         nodes = op2.Set(5)
-        elem_node = op2.Map(elements, nodes, 3, [0, 4, 3, 0, 1, 4, 1, 2, 4, 2, 3, 4])
+        elem_node = op2.Map(elements, nodes, 3, SPARSITY_FIXTURE['elem_node'])
         sparsity = op2.Sparsity((elem_node, elem_node))
-        assert list(sparsity._rowptr) == [0, 4, 8, 12, 16, 21]
-        assert list(sparsity._colidx) == [0, 1, 3, 4, 0, 1, 2, 4, 1, 2, 3, 4, 0, 2, 3, 4,
-                                          0, 1, 2, 3, 4]
+        assert list(sparsity._rowptr) == SPARSITY_FIXTURE['rowptr']
+        assert list(sparsity._colidx) == SPARSITY_FIXTURE['colidx']
 
     def test_sparsity_null_maps(self, backend):
         s = op2.Set(5)
@@ -227,11 +221,12 @@ class TestMatrices:
     def test_zero_rows(self, mesh):
         mat = mass_mat(mesh)
         expected = EXPECTED_MATRIX.copy()
-        expected[0] = [12.0, 0.0, 0.0, 0.0]
-        mat.zero_rows([0], 12.0)
+        first, last = ZERO_ROWS['first'], ZERO_ROWS['last']
+        expected[first['row']] = first['value']
+        mat.zero_rows([first['row']], first['diag'])
         assert_allclose(mat.values, expected, 1e-5)
-        expected[3] = [0.0, 0.0, 0.0, 4.0]
-        mat.zero_rows([3], 4.0)
+        expected[last['row']] = last['value']
+        mat.zero_rows([last['row']], last['diag'])
         assert_allclose(mat.values, expected, 1e-5)
 
     def test_assemble_vec_mass(self, mesh):
