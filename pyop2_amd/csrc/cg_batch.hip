@@ -25,8 +25,9 @@
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kChunk = 256;   // rows per chunk (one row per thread)
+constexpr int kChunk = 256;   // rows per chunk (kChunk / kBlock rows per thread)
 constexpr int kLanes = 8;     // lanes per row in SpMV
+constexpr int kRBlock = 1024; // workgroup of the per-system reductions of chunk partials
 
 enum { SYS_EMPTY = 0, SYS_NEW = 1, SYS_RUNNING = 2, SYS_DONE = 3 };
 
@@ -79,6 +80,8 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// sum over a workgroup of NB threads; the result is valid in thread 0
+template <int NB>
 __device__ inline double block_sum(double v, double* sh) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -86,7 +89,7 @@ __device__ inline double block_sum(double v, double* sh) {
   __syncthreads();
   double t = 0.0;
   if (threadIdx.x == 0)
-    for (int k = 0; k < kBlock / 64; ++k) t += sh[k];
+    for (int k = 0; k < NB / 64; ++k) t += sh[k];
   __syncthreads();
   return t;
 }
@@ -112,23 +115,32 @@ __global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunk
   int lo, hi;
   chunk_rows(s, cr.chunk, lo, hi);
   double zz = 0.0, zr = 0.0;
-  const int i = lo + threadIdx.x;
-  if (i < hi) {
+  // the system's arrays as local pointers: stores through them cannot
+  // change the descriptor, so it is read once
+  const int* __restrict__ rowptr = s.rowptr;
+  const int* __restrict__ colidx = s.colidx;
+  const double* __restrict__ vals = s.vals;
+  const double* __restrict__ b = s.b;
+  double* __restrict__ x = s.x;
+  double* __restrict__ inv_diag = s.inv_diag;
+  double* __restrict__ r = s.r;
+  double* __restrict__ z = s.z;
+  for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) {
     double d = 0.0;
-    for (int k = s.rowptr[i]; k < s.rowptr[i + 1]; ++k)
-      if (s.colidx[k] == i) d = s.vals[k];
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if (colidx[k] == i) d = vals[k];
     double inv = d != 0.0 ? 1.0 / d : 1.0;
-    s.inv_diag[i] = inv;
-    double ri = s.b[i];
+    inv_diag[i] = inv;
+    double ri = b[i];
     double zi = inv * ri;
-    s.x[i] = 0.0;
-    s.r[i] = ri;
-    s.z[i] = zi;
-    zz = zi * zi;
-    zr = zi * ri;
+    x[i] = 0.0;
+    r[i] = ri;
+    z[i] = zi;
+    zz += zi * zi;
+    zr += zi * ri;
   }
-  double t = block_sum(zz, sh);
-  double u = block_sum(zr, sh);
+  double t = block_sum<kBlock>(zz, sh);
+  double u = block_sum<kBlock>(zr, sh);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = t;
     part[2 * blockIdx.x + 1] = u;
@@ -138,12 +150,12 @@ __global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunk
 __device__ inline void reduce_pair(const Sys& s, const double* part, double* sh, double& a,
                                    double& b) {
   double va = 0.0, vb = 0.0;
-  for (int c = threadIdx.x; c < s.nchunks; c += kBlock) {
+  for (int c = threadIdx.x; c < s.nchunks; c += kRBlock) {
     va += part[2 * (s.chunk0 + c)];
     vb += part[2 * (s.chunk0 + c) + 1];
   }
-  a = block_sum(va, sh);
-  b = block_sum(vb, sh);
+  a = block_sum<kRBlock>(va, sh);
+  b = block_sum<kRBlock>(vb, sh);
 }
 
 __device__ inline void check(Sys& s) {
@@ -161,9 +173,9 @@ __device__ inline void check(Sys& s) {
 }
 
 // one workgroup per system
-__global__ void __launch_bounds__(kBlock) k_start(Sys* sys, const double* part,
+__global__ void __launch_bounds__(kRBlock) k_start(Sys* sys, const double* part,
                                                   double* reshist, int hist_stride) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kRBlock / 64];
   Sys& s = sys[blockIdx.x];
   if (s.state != SYS_NEW) return;
   double zz, zr;
@@ -192,13 +204,13 @@ __global__ void __launch_bounds__(kBlock) k_update_p(Sys* sys, const ChunkRef* c
   if (s.state != SYS_RUNNING) return;
   int lo, hi;
   chunk_rows(s, cr.chunk, lo, hi);
-  const int i = lo + threadIdx.x;
-  if (i >= hi) return;
+  double* __restrict__ p = s.p;
+  const double* __restrict__ z = s.z;
   if (s.its == 0) {
-    s.p[i] = s.z[i];
+    for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) p[i] = z[i];
   } else {
-    double bb = s.beta / s.betaold;
-    s.p[i] = s.z[i] + bb * s.p[i];
+    const double bb = s.beta / s.betaold;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) p[i] = z[i] + bb * p[i];
   }
 }
 
@@ -212,28 +224,33 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
   int lo, hi;
   chunk_rows(s, cr.chunk, lo, hi);
   const int sub = threadIdx.x & (kLanes - 1);
+  const int* __restrict__ rowptr = s.rowptr;
+  const int* __restrict__ colidx = s.colidx;
+  const double* __restrict__ vals = s.vals;
+  const double* __restrict__ p = s.p;
+  double* __restrict__ w = s.w;
   double acc = 0.0;
   for (int i = lo + (int)threadIdx.x / kLanes; i < hi; i += kBlock / kLanes) {
-    const int kb = s.rowptr[i], ke = s.rowptr[i + 1];
+    const int kb = rowptr[i], ke = rowptr[i + 1];
     double v = 0.0;
-    for (int k = kb + sub; k < ke; k += kLanes) v += s.vals[k] * s.p[s.colidx[k]];
+    for (int k = kb + sub; k < ke; k += kLanes) v += vals[k] * p[colidx[k]];
     for (int off = kLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kLanes);
     if (sub == 0) {
-      s.w[i] = v;
-      acc += s.p[i] * v;
+      w[i] = v;
+      acc += p[i] * v;
     }
   }
-  double t = block_sum(acc, sh);
+  double t = block_sum<kBlock>(acc, sh);
   if (threadIdx.x == 0) part[2 * blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(kBlock) k_alpha(Sys* sys, const double* part) {
-  __shared__ double sh[kBlock / 64];
+__global__ void __launch_bounds__(kRBlock) k_alpha(Sys* sys, const double* part) {
+  __shared__ double sh[kRBlock / 64];
   Sys& s = sys[blockIdx.x];
   if (s.state != SYS_RUNNING) return;
   double v = 0.0;
-  for (int c = threadIdx.x; c < s.nchunks; c += kBlock) v += part[2 * (s.chunk0 + c)];
-  double pw = block_sum(v, sh);
+  for (int c = threadIdx.x; c < s.nchunks; c += kRBlock) v += part[2 * (s.chunk0 + c)];
+  double pw = block_sum<kRBlock>(v, sh);
   if (threadIdx.x == 0) {
     s.its += 1;
     if (s.beta == 0.0) {
@@ -259,29 +276,34 @@ __global__ void __launch_bounds__(kBlock) k_axpy(Sys* sys, const ChunkRef* chunk
   if (s.state != SYS_RUNNING) return;
   int lo, hi;
   chunk_rows(s, cr.chunk, lo, hi);
-  const int i = lo + threadIdx.x;
   double zz = 0.0, zr = 0.0;
-  if (i < hi) {
-    const double a = s.alpha;
-    s.x[i] = s.x[i] + a * s.p[i];
-    double ri = s.r[i] - a * s.w[i];
-    s.r[i] = ri;
-    double zi = s.inv_diag[i] * ri;
-    s.z[i] = zi;
-    zz = zi * zi;
-    zr = zi * ri;
+  const double a = s.alpha;
+  double* __restrict__ x = s.x;
+  const double* __restrict__ p = s.p;
+  double* __restrict__ r = s.r;
+  const double* __restrict__ w = s.w;
+  const double* __restrict__ inv_diag = s.inv_diag;
+  double* __restrict__ z = s.z;
+  for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) {
+    x[i] = x[i] + a * p[i];
+    double ri = r[i] - a * w[i];
+    r[i] = ri;
+    double zi = inv_diag[i] * ri;
+    z[i] = zi;
+    zz += zi * zi;
+    zr += zi * ri;
   }
-  double t = block_sum(zz, sh);
-  double u = block_sum(zr, sh);
+  double t = block_sum<kBlock>(zz, sh);
+  double u = block_sum<kBlock>(zr, sh);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = t;
     part[2 * blockIdx.x + 1] = u;
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_finish(Sys* sys, const double* part, double* reshist,
+__global__ void __launch_bounds__(kRBlock) k_finish(Sys* sys, const double* part, double* reshist,
                                                    int hist_stride) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[kRBlock / 64];
   Sys& s = sys[blockIdx.x];
   if (s.state != SYS_RUNNING) return;
   double zz, zr;
@@ -393,15 +415,15 @@ int op2hip_cgb_step(void* h, int iters, void* stream) {
   OP2HIP_REQUIRE(b, "null batch");
   if (b->nsys == 0 || b->nchunks == 0) return OP2HIP_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  dim3 gc(static_cast<unsigned>(b->nchunks)), gs(b->nsys), blk(kBlock);
+  dim3 gc(static_cast<unsigned>(b->nchunks)), gs(b->nsys), blk(kBlock), rblk(kRBlock);
   hipLaunchKernelGGL(k_init, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
-  hipLaunchKernelGGL(k_start, gs, blk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
+  hipLaunchKernelGGL(k_start, gs, rblk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
   for (int it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_update_p, gc, blk, 0, s, b->d_sys, b->d_chunks);
     hipLaunchKernelGGL(k_spmv, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
-    hipLaunchKernelGGL(k_alpha, gs, blk, 0, s, b->d_sys, b->d_part);
+    hipLaunchKernelGGL(k_alpha, gs, rblk, 0, s, b->d_sys, b->d_part);
     hipLaunchKernelGGL(k_axpy, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
-    hipLaunchKernelGGL(k_finish, gs, blk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
+    hipLaunchKernelGGL(k_finish, gs, rblk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
   }
   OP2HIP_CHECK(hipGetLastError());
   return OP2HIP_OK;
