@@ -26,7 +26,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kChunk = 256;   // rows per chunk (kChunk / kBlock rows per thread)
-constexpr int kLanes = 8;     // lanes per row in SpMV
+constexpr int kLanes = 16;    // most lanes per row in SpMV
 constexpr int kRBlock = 1024; // workgroup of the per-system reductions of chunk partials
 
 enum { SYS_EMPTY = 0, SYS_NEW = 1, SYS_RUNNING = 2, SYS_DONE = 3 };
@@ -223,18 +223,25 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
   if (s.state != SYS_RUNNING) return;
   int lo, hi;
   chunk_rows(s, cr.chunk, lo, hi);
-  const int sub = threadIdx.x & (kLanes - 1);
   const int* __restrict__ rowptr = s.rowptr;
   const int* __restrict__ colidx = s.colidx;
   const double* __restrict__ vals = s.vals;
   const double* __restrict__ p = s.p;
   double* __restrict__ w = s.w;
+  // lanes per row from the chunk's mean row length (about two entries per
+  // lane: 4 lanes for 2-D P1 rows, 8 for 3-D, 16 for 3-D vector rows); a
+  // property of the chunk alone, so a system sums alike in any batch
+  const int nzc = rowptr[hi] - rowptr[lo];
+  const int mean = (nzc + (hi - lo) - 1) / (hi - lo);
+  int lanes = 2;
+  while (lanes < kLanes && 2 * lanes < mean) lanes <<= 1;
+  const int sub = threadIdx.x & (lanes - 1);
   double acc = 0.0;
-  for (int i = lo + (int)threadIdx.x / kLanes; i < hi; i += kBlock / kLanes) {
+  for (int i = lo + (int)threadIdx.x / lanes; i < hi; i += kBlock / lanes) {
     const int kb = rowptr[i], ke = rowptr[i + 1];
     double v = 0.0;
-    for (int k = kb + sub; k < ke; k += kLanes) v += vals[k] * p[colidx[k]];
-    for (int off = kLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kLanes);
+    for (int k = kb + sub; k < ke; k += lanes) v += vals[k] * p[colidx[k]];
+    for (int off = lanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, lanes);
     if (sub == 0) {
       w[i] = v;
       acc += p[i] * v;
