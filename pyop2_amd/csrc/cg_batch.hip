@@ -7,6 +7,9 @@
 //   r = b, z = D^-1 r, dp = |z|, test at it 0, beta = z.r
 //   loop: p = z (+ beta/betaold p); w = A p; dpi = p.w; a = beta/dpi;
 //         x += a p; r -= a w; z = D^-1 r; dp = |z|; test; beta = z.r
+// Four launches per iteration: w = A p with p formed from z and the old p
+// as it is gathered, partial p.w (k_spmv); a (k_alpha); p stored, x, r, z,
+// partials z.z and z.r (k_axpy); dp, the test and beta (k_finish).
 //   ttol = max(rtol*|z0|, atol); converged dp <= ttol; diverged
 //   dp >= dtol*|z0|; DIVERGED_ITS after max_it.
 //
@@ -198,21 +201,6 @@ __global__ void __launch_bounds__(kRBlock) k_start(Sys* sys, const double* part,
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_update_p(Sys* sys, const ChunkRef* chunks) {
-  const ChunkRef cr = chunks[blockIdx.x];
-  Sys& s = sys[cr.sys];
-  if (s.state != SYS_RUNNING) return;
-  int lo, hi;
-  chunk_rows(s, cr.chunk, lo, hi);
-  double* __restrict__ p = s.p;
-  const double* __restrict__ z = s.z;
-  if (s.its == 0) {
-    for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) p[i] = z[i];
-  } else {
-    const double bb = s.beta / s.betaold;
-    for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) p[i] = z[i] + bb * p[i];
-  }
-}
 
 // w = A p for the chunk's rows, partial p.w
 __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunks,
@@ -236,15 +224,28 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
   int lanes = 2;
   while (lanes < kLanes && 2 * lanes < mean) lanes <<= 1;
   const int sub = threadIdx.x & (lanes - 1);
+  const double* __restrict__ z = s.z;
+  // the new direction p' = z (first iteration) or z + (beta/betaold) p,
+  // formed where it is read: the gathered columns here, each row again in
+  // k_axpy, which stores it
+  const bool first = s.its == 0;
+  const double bb = first ? 0.0 : s.beta / s.betaold;
   double acc = 0.0;
   for (int i = lo + (int)threadIdx.x / lanes; i < hi; i += kBlock / lanes) {
     const int kb = rowptr[i], ke = rowptr[i + 1];
     double v = 0.0;
-    for (int k = kb + sub; k < ke; k += lanes) v += vals[k] * p[colidx[k]];
+    if (first) {
+      for (int k = kb + sub; k < ke; k += lanes) v += vals[k] * z[colidx[k]];
+    } else {
+      for (int k = kb + sub; k < ke; k += lanes) {
+        const int c = colidx[k];
+        v += vals[k] * (z[c] + bb * p[c]);
+      }
+    }
     for (int off = lanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, lanes);
     if (sub == 0) {
       w[i] = v;
-      acc += p[i] * v;
+      acc += (first ? z[i] : z[i] + bb * p[i]) * v;
     }
   }
   double t = block_sum<kBlock>(acc, sh);
@@ -286,13 +287,18 @@ __global__ void __launch_bounds__(kBlock) k_axpy(Sys* sys, const ChunkRef* chunk
   double zz = 0.0, zr = 0.0;
   const double a = s.alpha;
   double* __restrict__ x = s.x;
-  const double* __restrict__ p = s.p;
+  double* __restrict__ p = s.p;
   double* __restrict__ r = s.r;
   const double* __restrict__ w = s.w;
   const double* __restrict__ inv_diag = s.inv_diag;
   double* __restrict__ z = s.z;
+  // k_alpha has counted this iteration: its == 1 on the first
+  const bool first = s.its == 1;
+  const double bb = first ? 0.0 : s.beta / s.betaold;
   for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) {
-    x[i] = x[i] + a * p[i];
+    const double pi = first ? z[i] : z[i] + bb * p[i];
+    p[i] = pi;
+    x[i] = x[i] + a * pi;
     double ri = r[i] - a * w[i];
     r[i] = ri;
     double zi = inv_diag[i] * ri;
@@ -426,7 +432,6 @@ int op2hip_cgb_step(void* h, int iters, void* stream) {
   hipLaunchKernelGGL(k_init, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
   hipLaunchKernelGGL(k_start, gs, rblk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k_update_p, gc, blk, 0, s, b->d_sys, b->d_chunks);
     hipLaunchKernelGGL(k_spmv, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
     hipLaunchKernelGGL(k_alpha, gs, rblk, 0, s, b->d_sys, b->d_part);
     hipLaunchKernelGGL(k_axpy, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
