@@ -575,6 +575,9 @@ class Const(DataCarrier):
         """Const names are globally scoped and must be unique."""
 
     _defs = set()
+    # bumped when a Const is declared or removed; the sorted list is cached
+    _defs_version = 0
+    _sorted = (-1, [])
     _globalcount = 0
 
     @validate_type(('name', str, NameTypeError))
@@ -587,6 +590,7 @@ class Const(DataCarrier):
             raise Const.NonUniqueNameError(
                 "OP2 Constants are globally scoped, %s is already in use" % self._name)
         Const._defs.add(self)
+        Const._defs_version += 1
         Const._globalcount += 1
         self._version = 0
 
@@ -610,11 +614,14 @@ class Const(DataCarrier):
 
     @classmethod
     def _definitions(cls):
-        return sorted(Const._defs, key=lambda c: c.name)
+        if Const._sorted[0] != Const._defs_version:
+            Const._sorted = (Const._defs_version, sorted(Const._defs, key=lambda c: c.name))
+        return list(Const._sorted[1])
 
     def remove_from_namespace(self):
         """Remove this Const so the name can be declared again."""
         Const._defs.discard(self)
+        Const._defs_version += 1
 
     def _format_declaration(self):
         d = {'type': self.ctype, 'name': self.name, 'dim': self.cdim}

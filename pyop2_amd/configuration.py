@@ -53,6 +53,12 @@ class ConfigModule(types.ModuleType):
     """Module object answering ``cfg[key]`` and ``cfg.key``."""
 
     _config = None
+    # bumped on every change, so callers can cache values derived from the
+    # configuration
+    _version = 0
+
+    def version(self):
+        return self._version
 
     def configure(self, **kwargs):
         entries = dict(DEFAULTS)
@@ -71,32 +77,18 @@ class ConfigModule(types.ModuleType):
                 pass
         entries.update(kwargs)
         self._config = entries
+        self._version += 1
         if entries.get('debug', 0) > 0:
             warnings.simplefilter("always")
 
     def reset(self):
         """Forget all configuration entries."""
         self._config = None
+        self._version += 1
 
     def override(self, **kwargs):
         """Context manager setting entries for the duration of a block."""
-        mod = self
-
-        class _Override(object):
-            def __enter__(self):
-                if mod._config is None:
-                    mod._config = dict(DEFAULTS)
-                self.saved = {k: mod._config.get(k, _MISSING) for k in kwargs}
-                mod._config.update(kwargs)
-
-            def __exit__(self, *exc):
-                for k, v in self.saved.items():
-                    if v is _MISSING:
-                        mod._config.pop(k, None)
-                    else:
-                        mod._config[k] = v
-                return False
-        return _Override()
+        return _Override(self, kwargs)
 
     def get(self, key, default=None):
         """Return an option, falling back to the defaults before configure()."""
@@ -116,6 +108,33 @@ class ConfigModule(types.ModuleType):
             return self._config[name]
         except KeyError:
             raise AttributeError(name)
+
+
+class _Override(object):
+    """Entries of a configuration set for the duration of a block."""
+
+    def __init__(self, mod, kwargs):
+        self.mod, self.kwargs = mod, kwargs
+
+    def __enter__(self):
+        mod, kwargs = self.mod, self.kwargs
+        if mod._config is None:
+            mod._config = dict(DEFAULTS)
+        self.saved = {k: mod._config.get(k, _MISSING) for k in kwargs}
+        mod._config.update(kwargs)
+        if kwargs:
+            mod._version += 1
+
+    def __exit__(self, *exc):
+        mod = self.mod
+        for k, v in self.saved.items():
+            if v is _MISSING:
+                mod._config.pop(k, None)
+            else:
+                mod._config[k] = v
+        if self.saved:
+            mod._version += 1
+        return False
 
 
 sys.modules[__name__].__class__ = ConfigModule

@@ -73,8 +73,13 @@ def device():
     return _state['device']
 
 
+_raw_stream = getattr(torch._C, '_cuda_getCurrentRawStream', None)
+
+
 def stream():
     """The HIP stream every loop is launched on: torch's current stream."""
+    if _raw_stream is not None and _state['initialised']:
+        return _raw_stream(0)
     return torch.cuda.current_stream(device()).cuda_stream
 
 
@@ -1144,25 +1149,27 @@ class ParLoop(base.ParLoop):
 
     def _call_key(self):
         return (id(self.kernel), self._it_space.extents, id(self._it_space.iterset),
-                tuple(cfg.get(k) for k in _VARIANT_KEYS),
-                tuple(_arg_key(a) for a in self.args),
-                tuple((c.name, id(c)) for c in base.Const._definitions()))
+                _variant_key(), tuple(_arg_key(a) for a in self.args), _const_key())
 
     def _entry_cached(self):
         """The cached launch entry of this loop under the current (and, with
         the autotuner on, tuned) configuration."""
-        params = {}
         if cfg.get('hip_autotune'):
             mode = self._mode()
             params = autotune.params_for(self, mode, lambda cand: _trial(self, mode, cand))
-        with cfg.override(**params):
-            key = self._call_key()
-            ent = _call_cache.get(key)
-            if ent is None:
-                if len(_call_cache) >= _CALL_CACHE_MAX:
-                    _call_cache.pop(next(iter(_call_cache)))
-                ent = self._entry(self._mode())
-                _call_cache[key] = ent
+            if params:
+                with cfg.override(**params):
+                    return self._entry_for_key()
+        return self._entry_for_key()
+
+    def _entry_for_key(self):
+        key = self._call_key()
+        ent = _call_cache.get(key)
+        if ent is None:
+            if len(_call_cache) >= _CALL_CACHE_MAX:
+                _call_cache.pop(next(iter(_call_cache)))
+            ent = self._entry(self._mode())
+            _call_cache[key] = ent
         return ent
 
     def compute(self):
@@ -1479,6 +1486,31 @@ def run_chained(loops, launch=True):
         else:
             ent.prepare()
         i += len(grp)
+
+
+_key_cache = {'cfg': (None, None), 'const': (None, None)}
+
+
+def _variant_key():
+    """The configuration entries of _VARIANT_KEYS, recomputed only when the
+    configuration changed."""
+    v = cfg.version()
+    hit = _key_cache['cfg']
+    if hit[0] != v:
+        hit = (v, tuple(cfg.get(k) for k in _VARIANT_KEYS))
+        _key_cache['cfg'] = hit
+    return hit[1]
+
+
+def _const_key():
+    """(name, identity) of the declared Consts, recomputed only when one is
+    declared or removed."""
+    v = base.Const._defs_version
+    hit = _key_cache['const']
+    if hit[0] != v:
+        hit = (v, tuple((c.name, id(c)) for c in base.Const._definitions()))
+        _key_cache['const'] = hit
+    return hit[1]
 
 
 # configuration entries that shape a loop's launches (part of the call key)

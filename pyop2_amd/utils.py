@@ -62,25 +62,28 @@ class _validate_base(object):
         code = f.__code__
         where = (code.co_filename, code.co_firstlineno + 1)
 
+        # (name, position, has default, default, condition, exception) of
+        # each check on a parameter the function has, resolved once
+        checks = [(a, names.index(a), a in defaults, defaults.get(a), cond, exc)
+                  for a, cond, exc in self._checks if a in names]
+        check_arg = self.check_arg
+
         @functools.wraps(f)
         def wrapper(*args, **kwargs):
-            for argname, cond, exc in self._checks:
-                if argname not in names:
-                    continue
-                i = names.index(argname)
+            for argname, i, has_default, default, cond, exc in checks:
                 if argname in kwargs:
                     arg = kwargs[argname]
                 elif i < len(args):
                     arg = args[i]
                 else:
                     continue
-                if argname in defaults:
+                if has_default:
                     try:
-                        if arg is defaults[argname] or arg == defaults[argname]:
+                        if arg is default or arg == default:
                             continue
                     except Exception:
                         pass
-                self.check_arg(arg, cond, exc, where)
+                check_arg(arg, cond, exc, where)
             return f(*args, **kwargs)
         return wrapper
 
