@@ -16,9 +16,17 @@ from .logger import warning
 backends = {'void': void, 'finalised': finalised}
 
 
+_selectors = {}
+
+
 def _make_object(obj, *args, **kwargs):
-    """Instantiate the backend class named ``obj``."""
-    return _BackendSelector(obj, (object,), {})(*args, **kwargs)
+    """Instantiate the backend class named ``obj`` (the selector resolves
+    the backend at each call, so one selector class per name serves every
+    backend)."""
+    sel = _selectors.get(obj)
+    if sel is None:
+        sel = _selectors[obj] = _BackendSelector(obj, (object,), {})
+    return sel(*args, **kwargs)
 
 
 class _BackendSelector(type):
