@@ -154,13 +154,16 @@ def uniquify(iterable):
 
 def parser(description=None, group=False):
     """The default argument parser of op2 programs (reference:
-    utils.py:198-224), with ``hip`` among the backend choices."""
+    utils.py:198-224); the backend choices are the registered backends,
+    ``hip`` by default."""
     p = argparse.ArgumentParser(description=description, add_help=True,
                                 prefix_chars="-",
                                 formatter_class=argparse.RawDescriptionHelpFormatter)
     g = p.add_argument_group('pyop2', 'backend configuration options') if group else p
-    g.add_argument('-b', '--backend', default='hip',
-                   choices=['sequential', 'openmp', 'hip'],
+    from . import backends
+    names = sorted(set(['hip']) | set(k for k in backends.backends
+                                      if k not in ('void', 'finalised')))
+    g.add_argument('-b', '--backend', default='hip', choices=names,
                    help='select backend' if group else 'select pyop2 backend')
     g.add_argument('-d', '--debug', default=argparse.SUPPRESS, type=int,
                    choices=range(8),
