@@ -26,7 +26,7 @@ CANDIDATES = {
     'owner': [{'hip_owner_block_size': b, 'hip_owner_fill': f}
               for b in (256, 512) for f in (0.6, 0.9, 1.2)],
     'owner_mat': [{'hip_owner_mat_block_size': b, 'hip_owner_mat_fill': f}
-                  for b in (128, 256, 512) for f in (0.4, 0.6, 0.9)],
+                  for b in (128, 256, 512) for f in (0.4, 0.6, 0.7, 0.9)],
     'coloured': [{'hip_partition_size': p, 'hip_block_size': b}
                  for p in (128, 256, 512, 1024) for b in (128, 256) if b <= p],
 }

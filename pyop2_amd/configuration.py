@@ -29,7 +29,7 @@ DEFAULTS = {
     'hip_owner_fill': 0.9,        # owner blocks run about this many elements per thread
     'hip_owner_block_size': 512,  # workgroup size of owner-computes loops (0: hip_block_size)
     'hip_owner_mat_block_size': 256,  # ... of owner-computes Mat assembly
-    'hip_owner_mat_fill': 0.6,    # elements per thread of owner-computes Mat assembly
+    'hip_owner_mat_fill': 0.7,    # elements per thread of owner-computes Mat assembly
     'hip_stage_reads': False,     # stage read-only multi-component direct Dats through LDS
     'hip_stage_writes': True,     # stage written ones through LDS (else whole-row vector stores)
     'hip_owner_stage': False,     # owner blocks stage their owned rows of gathered Dats in LDS
