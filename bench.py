@@ -59,6 +59,8 @@ def parse():
     p.add_argument('--owner-stage', type=int, default=-1,
                    help='owner blocks stage owned rows of gathered Dats in LDS (-1: default)')
     p.add_argument('--owner-block-size', type=int, default=0)
+    p.add_argument('--owner-fill', type=float, default=0,
+                   help='elements per thread owner blocks are sized for (0: the default)')
     p.add_argument('--owner-prefetch', type=int, default=-1,
                    help='owner write-back loads issued before the elements (-1: default)')
     p.add_argument('--owner-waves', type=int, default=-1,
@@ -138,6 +140,8 @@ def main():
         opts['hip_stream_blocks'] = args.stream_blocks
     if args.owner_block_size:
         opts['hip_owner_block_size'] = args.owner_block_size
+    if args.owner_fill:
+        opts['hip_owner_fill'] = args.owner_fill
     if args.owner_targets:
         opts['hip_owner_targets'] = args.owner_targets
     op2.init(**opts)

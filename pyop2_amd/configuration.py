@@ -26,7 +26,7 @@ DEFAULTS = {
     'hip_autotune': False,        # tune launch parameters per loop and mesh-size bucket
     'hip_stream_blocks': 4096,    # most workgroups of a direct loop (more rows: grid-stride)
     'hip_stream_unroll': 1,       # elements per thread and chunk of direct loops (1 or 2)
-    'hip_owner_fill': 0.9,        # owner blocks run about this many elements per thread
+    'hip_owner_fill': 1.0,        # owner blocks run about this many elements per thread
     'hip_owner_block_size': 512,  # workgroup size of owner-computes loops (0: hip_block_size)
     'hip_owner_mat_block_size': 256,  # ... of owner-computes Mat assembly
     'hip_owner_mat_fill': 0.7,    # elements per thread of owner-computes Mat assembly

@@ -750,7 +750,7 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None,
             # elements run and LDS bytes per owned target, from the trial
             per_target = (p.nexec.sum() / float(p.nactive)) / kk
             lds_per_target = lds_of(p) / float(kk)
-            fill = cfg.get('hip_owner_fill') or 0.9
+            fill = cfg.get('hip_owner_fill') or 1.0
             kk = min(1 << 16, max(1, int(fill * block / max(per_target, 1e-9))))
             kk = max(1, min(kk, int(limit / max(lds_per_target, 1e-9))))
             p = build(kk)
