@@ -89,7 +89,11 @@ int op2hip_stream_sync(void* stream);
  * (row-major, dims[i] columns, at least nelems rows); idxs[i] the column
  * used (-1 all); spaces[i] the target index space of map i, of size
  * space_size[spaces[i]].  reorder = 1 orders the iteration set into compact
- * regions grown breadth-first through the race maps before partitioning. */
+ * regions grown breadth-first through the race maps before partitioning
+ * (two-level colouring: blocks, then threads inside a block, as plan.pyx);
+ * reorder = 2 colours the elements themselves (greedy, in element order) and
+ * cuts each colour's run into blocks: one thread colour per block, one
+ * block colour per element colour. */
 int op2hip_plan_build(int nelems, int part_size, int reorder, int nmaps, const int* const* maps,
                       const int* dims, const int* idxs, const int* spaces, int nspaces,
                       const int* space_size, void** plan_out);

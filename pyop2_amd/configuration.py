@@ -23,6 +23,9 @@ DEFAULTS = {
     'hip_partition_size': 0,      # 0: taken from the autotuner table
     'hip_block_size': 0,          # 0: taken from the autotuner table
     'hip_reorder': True,          # locality-ordered partitions for indirect loops
+    'hip_colouring': 'auto',      # coloured loops: 'elements' (one colour per element,
+                                  # blocks of one colour), 'blocks' (block and thread
+                                  # colours) or 'auto' (elements for Mat assembly)
     'hip_autotune': False,        # tune launch parameters per loop and mesh-size bucket
     'hip_stream_blocks': 4096,    # most workgroups of a direct loop (more rows: grid-stride)
     'hip_stream_unroll': 1,       # elements per thread and chunk of direct loops (1 or 2)

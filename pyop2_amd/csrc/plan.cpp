@@ -227,6 +227,44 @@ int op2hip_plan_build(int nelems, int part_size, int reorder, int nmaps, const i
   }
   Plan* p = new Plan();
   p->nelems = nelems;
+  if (reorder == 2 && !rmaps.empty() && nelems > 0) {
+    // element colouring: every element gets the first colour none of its
+    // targets carries yet (elements in order), elements are grouped by
+    // colour (ascending element order within a colour) and each colour's
+    // run is cut into blocks of part_size.  All blocks of a colour are
+    // conflict-free together and inside, so each block has one thread
+    // colour and a launch per colour runs without intra-block rounds.
+    std::vector<std::vector<uint64_t>> emasks(nspaces);
+    for (int s = 0; s < nspaces; ++s) emasks[s].assign(ssize[s], 0);
+    std::vector<int> ecol;
+    const int nc = greedy_colour(
+        nelems, emasks, [&](int it, auto&& cb) { for_targets(rmaps, it, cb); }, ecol);
+    std::vector<int> cstart(nc + 1, 0);
+    for (int e = 0; e < nelems; ++e) cstart[ecol[e] + 1]++;
+    for (int c = 0; c < nc; ++c) cstart[c + 1] += cstart[c];
+    std::vector<int> cur(cstart.begin(), cstart.end() - 1);
+    p->elems.resize(nelems);
+    for (int e = 0; e < nelems; ++e) p->elems[cur[ecol[e]]++] = e;
+    p->ncolors = nc;
+    p->ncolblk.assign(nc, 0);
+    for (int c = 0; c < nc; ++c)
+      for (int s0 = cstart[c]; s0 < cstart[c + 1]; s0 += part_size) {
+        const int len = std::min(part_size, cstart[c + 1] - s0);
+        p->offset.push_back(s0);
+        p->nelem.push_back(len);
+        p->blkcol.push_back(c);
+        p->ncolblk[c]++;
+        p->max_nelem = std::max(p->max_nelem, len);
+      }
+    p->nblocks = static_cast<int>(p->offset.size());
+    p->thrcol.assign(nelems, 0);
+    p->nthrcol.assign(p->nblocks, 1);
+    p->max_thrcol = 1;
+    p->blkmap.resize(p->nblocks);
+    for (int b = 0; b < p->nblocks; ++b) p->blkmap[b] = b;
+    *plan_out = p;
+    return OP2HIP_OK;
+  }
   std::vector<int> starts;
   if (reorder && !rmaps.empty() && nelems > 0) {
     grow_regions(nelems, part_size, rmaps, ssize, p->elems, starts);

@@ -469,7 +469,7 @@ class Plan(object):
                       nat.int_array(sizes))
         h = ctypes.c_void_p()
         mp, dims, idxs, sp, nsp, ssz = self._args
-        nat.check(nat.lib.op2hip_plan_build(n, part_size, 1 if reorder else 0, len(race), mp, dims,
+        nat.check(nat.lib.op2hip_plan_build(n, part_size, int(reorder), len(race), mp, dims,
                                             idxs, sp, nsp, ssz, ctypes.byref(h)), 'plan build')
         try:
             v = ctypes.c_int64()
@@ -543,11 +543,18 @@ def _race_maps(args):
 
 
 def get_plan(iterset, n, args, part_size=None, reorder=None):
-    """The cached plan of a coloured loop over ``n`` elements of ``iterset``."""
+    """The cached plan of a coloured loop over ``n`` elements of ``iterset``:
+    element colours or the reference's two-level block/thread colouring
+    (``hip_colouring`` 'elements' or 'blocks'; 'auto', the default, colours
+    elements for Mat assembly, whose many thread colours per block would
+    serialise its addto, and blocks otherwise)."""
     if part_size is None:
         part_size = cfg.get('hip_partition_size') or 256
     if reorder is None:
         reorder = cfg.get('hip_reorder', True)
+        how = cfg.get('hip_colouring', 'auto')
+        if how == 'elements' or (how == 'auto' and any(a._is_mat for a in args)):
+            reorder = 2
     race = _race_maps(args)
     # one entry per distinct (map, idx, space)
     seen, uniq = set(), []
@@ -556,7 +563,7 @@ def get_plan(iterset, n, args, part_size=None, reorder=None):
         if k not in seen:
             seen.add(k)
             uniq.append(r)
-    key = (id(iterset), n, part_size, bool(reorder), tuple((id(r[0]), r[1], r[2]) for r in uniq))
+    key = (id(iterset), n, part_size, int(reorder), tuple((id(r[0]), r[1], r[2]) for r in uniq))
     p = _plan_cache.get(key)
     if p is None:
         p = Plan(n, part_size, reorder,
@@ -1519,7 +1526,7 @@ _VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill', 'hi
                  'hip_owner_targets', 'hip_owner', 'hip_partition_size', 'hip_reorder',
                  'hip_stage_reads', 'hip_stage_writes', 'hip_stream_unroll',
                  'hip_stream_blocks', 'hip_owner_prefetch', 'hip_owner_waves',
-                 'hip_owner_mat_lds')
+                 'hip_owner_mat_lds', 'hip_colouring')
 
 
 def _written_buffers(pl):

@@ -53,7 +53,8 @@ def check_plan(p, maps, n):
 
 @settings(max_examples=25, deadline=None)
 @given(n=st.integers(1, 600), ntarget=st.integers(1, 80), dim=st.integers(1, 4),
-       ps=st.sampled_from([8, 32, 64, 256]), reorder=st.booleans(), seed=st.integers(0, 1000))
+       ps=st.sampled_from([8, 32, 64, 256]), reorder=st.sampled_from([0, 1, 2]),
+       seed=st.integers(0, 1000))
 def test_random_maps(n, ntarget, dim, ps, reorder, seed):
     rng = np.random.default_rng(seed)
     v = rng.integers(0, ntarget, size=(n, dim)).astype(np.int32)
@@ -80,6 +81,25 @@ def test_all_elements_one_target():
     check_plan(p, [(v, 0, 'u')], n)
     assert p.ncolors == p.nblocks
     assert (p.nthrcol == p.nelems).all()
+
+
+def test_element_colouring():
+    """reorder = 2: elements coloured one by one, each colour's run cut into
+    blocks; every block has one thread colour, blocks follow colour order,
+    elements ascend within a colour, and the plan is conflict-free."""
+    m = meshes.QuadMesh(120, 60, seed=5)
+    race = [(m.pecell, 2, 0, 'cells', m.ncell), (m.pecell, 2, 1, 'cells', m.ncell)]
+    p = Plan(m.nedge, 256, 2, race)
+    check_plan(p, [(m.pecell, 0, 'cells'), (m.pecell, 1, 'cells')], m.nedge)
+    assert (p.nthrcol == 1).all() and (p.thrcol == 0).all()
+    assert p.ncolors <= 8
+    assert np.array_equal(p.blkmap, np.arange(p.nblocks))
+    assert (np.diff(p.blkcol) >= 0).all()
+    for c in range(p.ncolors):
+        blocks = np.nonzero(p.blkcol == c)[0]
+        els = np.concatenate([p.elems[p.offset[b]:p.offset[b] + p.nelems[b]] for b in blocks])
+        assert (np.diff(els) > 0).all()
+    assert np.array_equal(np.sort(p.elems), np.arange(m.nedge))
 
 
 @pytest.mark.parametrize('reorder', [True, False])
