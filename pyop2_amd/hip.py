@@ -172,6 +172,8 @@ _map_buffers = {}
 
 
 class Map(base.Map):
+    # Args of this backend built directly (no backend lookup per argument)
+    _arg_class = Arg
 
     def __init__(self, *args, **kwargs):
         base.Map.__init__(self, *args, **kwargs)
@@ -200,6 +202,7 @@ class Map(base.Map):
 
 
 class Dat(_DeviceData, base.Dat):
+    _arg_class = Arg
 
     def __init__(self, *args, **kwargs):
         base.Dat.__init__(self, *args, **kwargs)
@@ -207,6 +210,7 @@ class Dat(_DeviceData, base.Dat):
 
 
 class Global(_DeviceData, base.Global):
+    _arg_class = Arg
 
     def __init__(self, *args, **kwargs):
         base.Global.__init__(self, *args, **kwargs)
@@ -315,6 +319,7 @@ class Sparsity(base.Sparsity):
 class Mat(base.Mat):
     """FP64 CSR values on the device, aligned with the Sparsity's CSR
     (reference: petsc_base.py:80-120 createAIJWithArrays)."""
+    _arg_class = Arg
 
     def __init__(self, *args, **kwargs):
         base.Mat.__init__(self, *args, **kwargs)
