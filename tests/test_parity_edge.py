@@ -48,7 +48,43 @@ def test_vector_inc_with_min_max(hip, n, nt, seed):
     assert abs(h[1] - o[1]) <= 1e-12 * max(1.0, abs(o[1]))
 
 
-def test_two_inc_dats_coloured(hip):
+@pytest.mark.parametrize('colouring', ['blocks', 'elements'])
+def test_two_inc_dats_coloured(hip, colouring):
+    """Two INC Dats take the coloured template, with the reference's
+    block/thread colours or with element colours: within 1e-12."""
+    from pyop2_amd import configuration as cfg
+    with cfg.override(hip_colouring=colouring):
+        _two_inc_dats(hip)
+
+
+def test_mat_coloured_assembly(hip):
+    """Mat assembly forced onto the coloured template (element colours by
+    default for Mat loops, and block/thread colours): the P1 mass matrix
+    within 1e-12 of the oracle's."""
+    from pyop2_amd import configuration as cfg
+    from pyop2_amd.workloads import fem, meshes
+    mesh = meshes.TriMesh(12, seed=4)
+    ref = None
+    for colouring in (None, 'auto', 'blocks', 'elements'):
+        W = seq if colouring is None else hip
+        nodes = W.Set(mesh.nvert)
+        vn = W.Set(mesh.nvert, 2)
+        el = W.Set(mesh.nelem)
+        en = W.Map(el, nodes, 3, mesh.elem_node)
+        ev = W.Map(el, vn, 3, mesh.elem_node)
+        x = W.Dat(vn, mesh.coords.copy(), np.float64)
+        A = W.Mat(W.Sparsity((en, en)), np.float64)
+        with cfg.override(hip_owner=False, hip_colouring=colouring or 'auto'):
+            W.par_loop(W.Kernel(fem.MASS_TRI, 'mass_tri'), el(3, 3),
+                       A((en[W.i[0]], en[W.i[1]]), W.INC), x(ev, W.READ))
+        vals = np.array(A.array)
+        if ref is None:
+            ref = vals
+        else:
+            assert np.abs(vals - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def _two_inc_dats(hip):
     code = """void k(double *x, double *a, double *b) { *a += x[0]; *b += x[0] * x[0]; }"""
     rng = np.random.default_rng(5)
     n, nt = 4000, 700
