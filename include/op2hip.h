@@ -218,6 +218,10 @@ int op2hip_cgb_create(int max_sys, long long max_chunks, int hist_len, void** ou
 int op2hip_cgb_destroy(void* h);
 int op2hip_cgb_load(void* h, int nsys, const op2hip_cg_system* systems, void* stream);
 int op2hip_cgb_step(void* h, int iters, void* stream);
+/* The same step replayed from a hipGraph captured (on a stream of the
+ * batch's own) once per chunk list and iteration count: the batching
+ * layer's captured CG steps.  Results equal op2hip_cgb_step's. */
+int op2hip_cgb_step_graph(void* h, int iters, void* stream);
 int op2hip_cgb_read(void* h, op2hip_cg_system* out, void* stream);
 int op2hip_cgb_history(void* h, int k, double* out, void* stream);
 int op2hip_cgb_sizeof_system(void);

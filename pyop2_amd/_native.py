@@ -111,6 +111,7 @@ _SIGS = {
     'op2hip_cgb_destroy': (c_int, [c_void]),
     'op2hip_cgb_load': (c_int, [c_void, c_int, c_void, c_void]),
     'op2hip_cgb_step': (c_int, [c_void, c_int, c_void]),
+    'op2hip_cgb_step_graph': (c_int, [c_void, c_int, c_void]),
     'op2hip_cgb_read': (c_int, [c_void, c_void, c_void]),
     'op2hip_cgb_history': (c_int, [c_void, c_int, c_void, c_void]),
     'op2hip_cgb_sizeof_system': (c_int, []),

@@ -102,14 +102,14 @@ class MeshBatch(object):
                                       rtol, atol, dtol, maxit))
         return out
 
-    def solve(self, rtol=1e-10, maxit=10000, chunk=32):
+    def solve(self, rtol=1e-10, maxit=10000, chunk=32, graph=True):
         """Batched CG of every mesh to ``rtol``; returns per-mesh
         (iterations, residual norm, PETSc reason)."""
         from . import hip
         systems = self.systems(rtol=rtol, maxit=maxit)
         if self._cg is None or self._cg.max_sys < len(systems):
             self._cg = cg.CGBatch(len(systems), int(self.voff[-1]))
-        out = self._cg.solve(systems, hip.stream(), chunk=chunk)
+        out = self._cg.solve(systems, hip.stream(), chunk=chunk, graph=graph)
         self.x._device_written()
         return [(s.its, s.dp, s.reason) for s in out]
 

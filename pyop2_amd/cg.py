@@ -77,8 +77,11 @@ class CGBatch(object):
         nat.check(nat.lib.op2hip_cgb_load(self._h, n, ctypes.byref(self.systems),
                                           ctypes.c_void_p(stream)), 'cg batch load')
 
-    def step(self, iters, stream):
-        nat.check(nat.lib.op2hip_cgb_step(self._h, iters, ctypes.c_void_p(stream)), 'cg step')
+    def step(self, iters, stream, graph=False):
+        """Enqueue ``iters`` iterations; ``graph`` replays them from a
+        captured hipGraph (kept while the chunk list and count stay)."""
+        f = nat.lib.op2hip_cgb_step_graph if graph else nat.lib.op2hip_cgb_step
+        nat.check(f(self._h, iters, ctypes.c_void_p(stream)), 'cg step')
 
     def read(self, stream):
         nat.check(nat.lib.op2hip_cgb_read(self._h, ctypes.byref(self.systems),
@@ -91,25 +94,31 @@ class CGBatch(object):
                                              ctypes.c_void_p(stream)), 'cg history')
         return out
 
-    def solve(self, systems, stream, chunk=16):
+    def solve(self, systems, stream, chunk=16, graph=False):
         """Run ``systems`` (all NEW) to completion; returns their final
-        descriptors."""
+        descriptors.  With ``graph`` the steps are replayed hipGraphs of
+        ``chunk`` iterations, and finished systems leave the launches only
+        when they hold half of the rows launched (each reload re-captures)."""
         self.load(systems, stream)
         maxit = max(s.maxit for s in systems) if systems else 0
+        ch = chunk_rows()
+        live = lambda ss: sum((s.n + ch - 1) // ch for s in ss if s.state != SYS_DONE)
+        loaded = live(systems)
         done_iters = 0
         ndone = 0
         while True:
-            k = chunk if done_iters else min(chunk, 4)
-            self.step(k, stream)
+            k = chunk if (done_iters or graph) else min(chunk, 4)
+            self.step(k, stream, graph)
             done_iters += k
             out = self.read(stream)
             d = sum(s.state == SYS_DONE for s in out)
             if d == len(out) or done_iters > maxit + chunk:
                 return out
-            if d > ndone:
+            if d > ndone and (not graph or 2 * live(out) <= loaded):
                 # drop finished systems' rows from the launches
                 self.load(out, stream)
                 ndone = d
+                loaded = live(out)
 
     def __del__(self):
         try:

@@ -348,7 +348,32 @@ struct Batch {
   double* d_hist = nullptr;
   int hist_stride = 0;
   std::vector<Sys> h_sys;
+  // a captured step (graph of `g_iters` iterations for the current chunk
+  // list), rebuilt when the chunk list or the iteration count changes
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_iters = -1;
+  long long g_nchunks = -1;
+  int g_nsys = -1;
 };
+
+void enqueue_step(Batch* b, int iters, hipStream_t s) {
+  dim3 gc(static_cast<unsigned>(b->nchunks)), gs(b->nsys), blk(kBlock), rblk(kRBlock);
+  hipLaunchKernelGGL(k_init, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
+  hipLaunchKernelGGL(k_start, gs, rblk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_spmv, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
+    hipLaunchKernelGGL(k_alpha, gs, rblk, 0, s, b->d_sys, b->d_part);
+    hipLaunchKernelGGL(k_axpy, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
+    hipLaunchKernelGGL(k_finish, gs, rblk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
+  }
+}
+
+void drop_graph(Batch* b) {
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  b->g_iters = -1;
+}
 
 }  // namespace
 
@@ -382,6 +407,8 @@ int op2hip_cgb_create(int max_sys, long long max_chunks, int hist_len, void** ou
 int op2hip_cgb_destroy(void* h) {
   Batch* b = static_cast<Batch*>(h);
   if (!b) return OP2HIP_OK;
+  drop_graph(b);
+  if (b->cap) hipStreamDestroy(b->cap);
   hipFree(b->d_sys);
   hipFree(b->d_chunks);
   hipFree(b->d_part);
@@ -427,17 +454,35 @@ int op2hip_cgb_step(void* h, int iters, void* stream) {
   Batch* b = static_cast<Batch*>(h);
   OP2HIP_REQUIRE(b, "null batch");
   if (b->nsys == 0 || b->nchunks == 0) return OP2HIP_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  dim3 gc(static_cast<unsigned>(b->nchunks)), gs(b->nsys), blk(kBlock), rblk(kRBlock);
-  hipLaunchKernelGGL(k_init, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
-  hipLaunchKernelGGL(k_start, gs, rblk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
-  for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k_spmv, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
-    hipLaunchKernelGGL(k_alpha, gs, rblk, 0, s, b->d_sys, b->d_part);
-    hipLaunchKernelGGL(k_axpy, gc, blk, 0, s, b->d_sys, b->d_chunks, b->d_part);
-    hipLaunchKernelGGL(k_finish, gs, rblk, 0, s, b->d_sys, b->d_part, b->d_hist, b->hist_stride);
-  }
+  enqueue_step(b, iters, static_cast<hipStream_t>(stream));
   OP2HIP_CHECK(hipGetLastError());
+  return OP2HIP_OK;
+}
+
+// The same step replayed from a hipGraph: the 2 + 4 * iters launches are
+// captured once per chunk list and iteration count (on a stream of the
+// batch's own) and then launched on `stream` as one graph.  Results are
+// those of op2hip_cgb_step.
+int op2hip_cgb_step_graph(void* h, int iters, void* stream) {
+  Batch* b = static_cast<Batch*>(h);
+  OP2HIP_REQUIRE(b && iters >= 0, "bad batch step");
+  if (b->nsys == 0 || b->nchunks == 0) return OP2HIP_OK;
+  if (!b->gexec || b->g_iters != iters || b->g_nchunks != b->nchunks || b->g_nsys != b->nsys) {
+    drop_graph(b);
+    if (!b->cap) OP2HIP_CHECK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    OP2HIP_CHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    enqueue_step(b, iters, b->cap);
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    OP2HIP_CHECK(e);
+    e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    OP2HIP_CHECK(e);
+    b->g_iters = iters;
+    b->g_nchunks = b->nchunks;
+    b->g_nsys = b->nsys;
+  }
+  OP2HIP_CHECK(hipGraphLaunch(b->gexec, static_cast<hipStream_t>(stream)));
   return OP2HIP_OK;
 }
 

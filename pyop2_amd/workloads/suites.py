@@ -58,7 +58,7 @@ def batch_suite(op2, nmesh=64, emin=2000, emax=200000, rtol=1e-10, seed=0, round
     t = time.perf_counter()
     for i, s1 in enumerate(singles):
         s1.assemble()
-        s1.solve(rtol=rtol, chunk=16)
+        s1.solve(rtol=rtol, chunk=16, graph=False)
         if (i + 1) % 32 == 0:
             say('one-at-a-time warm-up: %d meshes' % (i + 1))
     torch.cuda.synchronize()
@@ -67,7 +67,7 @@ def batch_suite(op2, nmesh=64, emin=2000, emax=200000, rtol=1e-10, seed=0, round
     for _ in range(rounds):
         for i, s1 in enumerate(singles):
             s1.assemble()
-            s1.solve(rtol=rtol, chunk=16)
+            s1.solve(rtol=rtol, chunk=16, graph=False)
             if (i + 1) % 32 == 0:
                 say('one-at-a-time timed: %d meshes' % (i + 1))
     torch.cuda.synchronize()

@@ -62,3 +62,20 @@ def test_batch_matches_oracle(hip):
         ref = u.data_ro.reshape(-1)
         got = mix.solution(k)
         assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_batch_graph_steps_match_eager(hip):
+    """Replayed hipGraph steps (re-captured whenever finished systems leave
+    the launches) give the eager steps' solutions bit for bit and the same
+    iteration counts, also on a second solve that reuses the batch."""
+    ms, fs, ks = make_meshes([9, 25, 4, 14], seed=7)
+    mix = MeshBatch(hip, ms, fs, ks, name='gm')
+    mix.assemble()
+    eager = mix.solve(rtol=1e-10, chunk=8, graph=False)
+    xe = [mix.solution(k).copy() for k in range(len(ms))]
+    for _ in range(2):
+        got = mix.solve(rtol=1e-10, chunk=8, graph=True)
+        assert [o[0] for o in got] == [o[0] for o in eager]
+        assert [o[2] for o in got] == [o[2] for o in eager]
+        for k in range(len(ms)):
+            assert np.array_equal(mix.solution(k), xe[k])
