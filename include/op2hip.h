@@ -233,6 +233,23 @@ int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, 
                double* x, double* work, double rtol, double atol, double dtol, int maxit,
                int* its, double* rnorm, int* reason, void* stream);
 
+/* GMRES and BiCGStab for one CSR on the device (Solver.solve with
+ * linear_solver 'gmres' or 'bcgs'/'bicgstab', pyop2/petsc_base.py:172-220;
+ * the cusp backend's methods, pyop2/cuda.py:504-508): PETSc 3.3 KSPGMRES
+ * (restarted every `restart` steps, classical Gram-Schmidt) or KSPBCGS,
+ * with PCJACOBI on the left and the preconditioned residual norm.  method 1
+ * is GMRES, 2 BiCGStab.  op2hip_krylov_work gives the device workspace in
+ * doubles (-1 for bad arguments); x is overwritten (zero initial guess).
+ * reshist (device, hist_len doubles, may be NULL with hist_len 0) receives
+ * the monitor's residual norm per iteration.  its, rnorm and reason
+ * (KSPConvergedReason) are host outputs.  Waits for the stream. */
+long long op2hip_krylov_work(int method, int n, int restart);
+int op2hip_krylov_solve(int method, int n, const int* rowptr, const int* colidx,
+                        const double* vals, const double* b, double* x, double* work,
+                        long long work_len, int restart, double rtol, double atol, double dtol,
+                        int maxit, double* reshist, int hist_len, int* its, double* rnorm,
+                        int* reason, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

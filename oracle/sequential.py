@@ -190,28 +190,38 @@ class Mat(base.Mat):
         return self
 
 
+KRYLOV_NAMES = {'cg': 'pcg', 'gmres': 'gmres', 'bcgs': 'bcgs', 'bicgstab': 'bcgs'}
+
+
 class Solver(base.Solver):
-    """PETSc KSPCG + PCJACOBI restated (oracle_pcg)."""
+    """PETSc KSPCG, KSPGMRES and KSPBCGS with PCJACOBI restated (oracle_pcg,
+    oracle_gmres, oracle_bcgs); linear_solver 'cg', 'gmres', or 'bcgs' /
+    'bicgstab' (the reference's cusp name, pyop2/cuda.py:504-508)."""
 
     def solve(self, A, x, b):
         p = self.parameters
-        assert p['linear_solver'] == 'cg' and p['preconditioner'] == 'jacobi'
+        method = KRYLOV_NAMES.get(p['linear_solver'])
+        assert method is not None and p['preconditioner'] == 'jacobi'
         sp = A.sparsity
         n = sp.nrows * A.dims[0]
         bb = np.ascontiguousarray(b.data_ro, dtype=np.float64).reshape(-1)
         xx = np.zeros(n)
         maxit = int(p['maximum_iterations'])
-        hist = np.zeros(maxit + 1)
+        hist = np.zeros(maxit + 2)
         its = ctypes.c_int()
         rn = ctypes.c_double()
-        f = rt().oracle_pcg
+        tol = [float(p['relative_tolerance']), float(p['absolute_tolerance']),
+               float(p['divergence_tolerance'])]
+        head = [n, sp._rowptr.ctypes.data, sp._colidx.ctypes.data, A._array.ctypes.data,
+                bb.ctypes.data, xx.ctypes.data]
+        tail = [maxit, ctypes.byref(its), ctypes.byref(rn), hist.ctypes.data]
+        f = getattr(rt(), 'oracle_' + method)
         f.restype = ctypes.c_int
-        f.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_double] * 3 + \
-            [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-        reason = f(n, sp._rowptr.ctypes.data, sp._colidx.ctypes.data, A._array.ctypes.data,
-                   bb.ctypes.data, xx.ctypes.data, float(p['relative_tolerance']),
-                   float(p['absolute_tolerance']), float(p['divergence_tolerance']), maxit,
-                   ctypes.byref(its), ctypes.byref(rn), hist.ctypes.data)
+        extra = [int(p.get('gmres_restart', 30))] if method == 'gmres' else []
+        f.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] * len(extra) + \
+            [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+        reason = f(*(head + extra + tol + tail))
         x.data[:] = xx.reshape(x.data.shape)
         self._its, self._rnorm, self._reason = its.value, rn.value, reason
         self.reshist = list(hist[:its.value + 1])

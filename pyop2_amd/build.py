@@ -14,7 +14,7 @@ LIBDIR = os.path.join(HERE, 'lib')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('OP2HIP_ARCH', 'gfx950')
 
-SOURCES = ['runtime.hip', 'plan.cpp', 'sparsity.hip', 'linalg.hip', 'cg_batch.hip']
+SOURCES = ['runtime.hip', 'plan.cpp', 'sparsity.hip', 'linalg.hip', 'cg_batch.hip', 'krylov.hip']
 
 # Device code is built without packed FP32 instructions (some of their forms
 # have given wrong results on this hardware) and without FP contraction, so

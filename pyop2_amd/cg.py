@@ -14,8 +14,9 @@ from . import _native as nat
 
 SYS_EMPTY, SYS_NEW, SYS_RUNNING, SYS_DONE = 0, 1, 2, 3
 
-REASONS = {2: 'CONVERGED_RTOL', 3: 'CONVERGED_ATOL', -3: 'DIVERGED_ITS', -4: 'DIVERGED_DTOL',
-           -8: 'DIVERGED_INDEFINITE_MAT', -9: 'DIVERGED_NANORINF'}
+REASONS = {2: 'CONVERGED_RTOL', 3: 'CONVERGED_ATOL', -2: 'DIVERGED_NULL', -3: 'DIVERGED_ITS',
+           -4: 'DIVERGED_DTOL', -5: 'DIVERGED_BREAKDOWN', -8: 'DIVERGED_INDEFINITE_MAT',
+           -9: 'DIVERGED_NANORINF'}
 
 _p = ctypes.c_void_p
 _d = ctypes.c_double

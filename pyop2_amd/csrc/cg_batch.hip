@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "common.h"
+#include "reduce.h"
 
 namespace {
 
@@ -78,24 +79,7 @@ namespace {
 
 using Sys = op2hip_cg_system;
 
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// sum over a workgroup of NB threads; the result is valid in thread 0
-template <int NB>
-__device__ inline double block_sum(double v, double* sh) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int k = 0; k < NB / 64; ++k) t += sh[k];
-  __syncthreads();
-  return t;
-}
+using op2hip::block_sum;
 
 // chunk list entry: (system, chunk index within the system)
 struct ChunkRef {

@@ -388,20 +388,30 @@ class Mat(base.Mat):
         return self
 
 
+# linear_solver names: PETSc's ('cg', 'gmres', 'bcgs') and cusp's 'bicgstab'
+# (pyop2/cuda.py:504-508)
+_KRYLOV = {'cg': 0, 'gmres': 1, 'bcgs': 2, 'bicgstab': 2}
+
+
 class Solver(base.Solver):
-    """Jacobi-preconditioned CG on the device with PETSc KSPCG semantics
-    (reference: petsc_base.py:160-220)."""
+    """Jacobi-preconditioned Krylov solves on the device with PETSc 3.3
+    semantics (reference: petsc_base.py:160-220): KSPCG for 'cg', KSPGMRES
+    (restart ``gmres_restart``) for 'gmres', KSPBCGS for 'bcgs' or
+    'bicgstab'."""
 
     def solve(self, A, x, b):
         p = self.parameters
-        if p['linear_solver'] != 'cg':
-            raise NotImplementedError("linear_solver %r: the hip backend implements 'cg'"
-                                      % p['linear_solver'])
+        method = _KRYLOV.get(p['linear_solver'])
+        if method is None:
+            raise NotImplementedError("linear_solver %r: the hip backend implements %s"
+                                      % (p['linear_solver'], sorted(_KRYLOV)))
         if p['preconditioner'] not in ('jacobi',):
             raise NotImplementedError("preconditioner %r: the hip backend implements 'jacobi'"
                                       % p['preconditioner'])
         if p['plot_convergence']:
             p['monitor_convergence'] = True
+        if method:
+            return self._krylov(method, A, x, b)
         A._check_err()
         sp = A.sparsity
         n = sp.nrows * A.dims[0]
@@ -420,6 +430,44 @@ class Solver(base.Solver):
         its, rnorm, reason = out.its, out.dp, out.reason
         if hist_len:
             self.reshist = list(batch.history(0, s)[:its + 1])
+        self._finish(x, its, rnorm, reason)
+
+    def _krylov(self, method, A, x, b):
+        """GMRES (method 1) or BiCGStab (2) through op2hip_krylov_solve."""
+        p = self.parameters
+        A._check_err()
+        sp = A.sparsity
+        n = sp.nrows * A.dims[0]
+        restart = int(p.get('gmres_restart', 30))
+        if method == 1 and restart < 1:
+            raise ValueError("gmres_restart must be at least 1")
+        bdev = b._to_device()
+        xdev = x._to_device()
+        nwork = nat.lib.op2hip_krylov_work(method, n, restart)
+        if nwork < 0:
+            raise ValueError("bad Krylov solve arguments")
+        work = torch.empty(max(int(nwork), 1), dtype=torch.float64, device=device())
+        maxit = int(p['maximum_iterations'])
+        hist = None
+        if p['monitor_convergence']:
+            hist = torch.zeros(maxit + 1, dtype=torch.float64, device=device())
+        its, rnorm, reason = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        nat.check(nat.lib.op2hip_krylov_solve(
+            method, n, ctypes.c_void_p(_ptr(sp._dev_rowptr)), ctypes.c_void_p(_ptr(sp._dev_colidx)),
+            ctypes.c_void_p(_ptr(A._dev_vals)), ctypes.c_void_p(_ptr(bdev)),
+            ctypes.c_void_p(_ptr(xdev)), ctypes.c_void_p(_ptr(work)), int(work.numel()), restart,
+            float(p['relative_tolerance']), float(p['absolute_tolerance']),
+            float(p['divergence_tolerance']), maxit,
+            ctypes.c_void_p(_ptr(hist) if hist is not None else None),
+            0 if hist is None else int(hist.numel()), ctypes.byref(its), ctypes.byref(rnorm),
+            ctypes.byref(reason), ctypes.c_void_p(stream())), 'krylov solve')
+        its, rnorm, reason = its.value, rnorm.value, reason.value
+        if hist is not None:
+            self.reshist = list(hist[:its + 1].cpu().numpy()) if n else [0.0]
+        self._finish(x, its, rnorm, reason)
+
+    def _finish(self, x, its, rnorm, reason):
+        p = self.parameters
         x._device_written()
         x.needs_halo_update = True
         self._its, self._rnorm, self._reason = its, rnorm, reason

@@ -1,12 +1,21 @@
-"""Time the device CG on one large 2-D system.
+"""Time the device Krylov solvers on one large 2-D system.
 
     python3 -m pyop2_amd.workloads.cgbench [elements] [iterations]
 
 Builds a seeded triangle mesh of about ``elements`` elements (default 2M),
 assembles A = M + K and b = M f with the batching layer's loops, then runs
-``iterations`` (default 300) Jacobi-PCG iterations (tolerance 1e-30, so
-none stop early) and prints one JSON line: time per iteration and the
-bytes one iteration moves (12 nnz + 108 n) over that time.
+``iterations`` (default 300) iterations (tolerance 1e-30, so none stop
+early) of Jacobi-PCG, GMRES(30) and BiCGStab, and prints one JSON line per
+method: time per iteration and the algorithmic bytes of an iteration over
+that time.  Bytes per iteration (n rows, nnz entries, 8-byte values, 4-byte
+indices):
+  CG        12 nnz + 108 n  (as fused: SpMV with the direction formed where
+            gathered, and one update pass)
+  GMRES(m)  12 nnz + 28 n for the SpMV with D^-1, then 8 n (j + 2) for the
+            Gram-Schmidt dots and 8 n (j + 3) for the update of step j;
+            averaged over j = 0 .. m-1
+  BiCGStab  24 nnz + 168 n  (direction 32 n, two SpMV 12 nnz + 36 n and
+            12 nnz + 44 n, update 56 n)
 """
 
 import json
@@ -31,10 +40,27 @@ def main(argv):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t
     n, nnz, done = mesh.nvert, B.sparsity.nz, r[0][0]
-    per_it = 12 * nnz + 108 * n
-    print(json.dumps({'elements': mesh.nelem, 'rows': n, 'nnz': nnz, 'iterations': done,
-                      'us_per_iteration': round(dt / done * 1e6, 2),
-                      'bytes_per_iteration': per_it,
+    report('cg', mesh, n, nnz, done, dt, 12 * nnz + 108 * n)
+    m = 30
+    gm = 12 * nnz + 28 * n + sum(8 * n * (j + 2) + 8 * n * (j + 3) for j in range(m)) / m
+    for method, per_it in (('gmres', gm), ('bcgs', 24 * nnz + 168 * n)):
+        solver = op2.Solver(linear_solver=method, relative_tolerance=1e-30,
+                            maximum_iterations=20, error_on_nonconvergence=False,
+                            gmres_restart=m)
+        solver.solve(B.A, B.x, B.b)
+        solver.parameters['maximum_iterations'] = its
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        solver.solve(B.A, B.x, B.b)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        report(method, mesh, n, nnz, solver.getIterationNumber(), dt, per_it)
+
+
+def report(method, mesh, n, nnz, done, dt, per_it):
+    print(json.dumps({'method': method, 'elements': mesh.nelem, 'rows': n, 'nnz': nnz,
+                      'iterations': done, 'us_per_iteration': round(dt / done * 1e6, 2),
+                      'bytes_per_iteration': int(per_it),
                       'GBps': round(per_it * done / dt / 1e9, 1)}), flush=True)
 
 

@@ -183,6 +183,18 @@ class TestMatrices:
         op2.solve(mat, x, b)
         assert_allclose(x.data, mesh['f'].data, 1e-8)
 
+    @pytest.mark.parametrize('method', ['gmres', 'bcgs', 'bicgstab'])
+    def test_solve_krylov(self, mesh, method):
+        """The same solve through GMRES and BiCGStab (the reference's demos
+        ask for GMRES, demo/laplace_ffc.py:151)."""
+        mat = mass_mat(mesh)
+        b = rhs(mesh)
+        x = op2.Dat(mesh['nodes'], np.zeros(4), np.float64, "x")
+        solver = op2.Solver(linear_solver=method, relative_tolerance=1e-12)
+        solver.solve(mat, x, b)
+        assert solver.getConvergedReason() > 0
+        assert_allclose(x.data, mesh['f'].data, 1e-8)
+
     def test_zero_matrix(self, mesh):
         mat = mass_mat(mesh)
         mat.zero()
