@@ -75,9 +75,9 @@ struct Prob {
   double* x;
   double* inv;
   State* st;
-  double* part;  // max(nb, nbs) rows of ps partials
+  double* part;  // partials by quantity: quantity k of workgroup b at k * ps + b
   double* hist;  // residual history (hist_len) or null
-  int n, lanes, ps, m, maxit, hist_len;
+  int n, lanes, ps, m, maxit, hist_len;  // ps = max(nb, nbs)
   int nb;   // workgroups of the tile and element kernels (kTile rows each)
   int nbs;  // workgroups of the SpMV kernels (more, for gather latency)
   // GMRES
@@ -115,9 +115,8 @@ __device__ inline void reduce_parts(const Prob& p, int nparts, int k0, double* o
 #pragma unroll
   for (int q = 0; q < K; ++q) v[q] = 0.0;
   for (int b = threadIdx.x; b < nparts; b += kRBlock) {
-    const double* row = p.part + (size_t)b * p.ps + k0;
 #pragma unroll
-    for (int q = 0; q < K; ++q) v[q] += row[q];
+    for (int q = 0; q < K; ++q) v[q] += p.part[(size_t)(k0 + q) * p.ps + b];
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -145,7 +144,7 @@ __device__ inline double reduce_part(const Prob& p, int nparts, int k, double* s
 
 __device__ inline void store_part(const Prob& p, int k, double v, double* sh) {
   double t = block_sum<kBlock>(v, sh);
-  if (threadIdx.x == 0) p.part[(size_t)blockIdx.x * p.ps + k] = t;
+  if (threadIdx.x == 0) p.part[(size_t)k * p.ps + blockIdx.x] = t;
 }
 
 #define ROWS(i) for (int i = blockIdx.x * kBlock + threadIdx.x; i < p.n; i += p.nb * kBlock)
@@ -257,7 +256,25 @@ __global__ void __launch_bounds__(kBlock) k_g_dots(Prob p) {
     const int i = base + r * kBlock;
     wv[r] = i < p.n ? w[i] : 0.0;
   }
-  for (int k = 0; k < j; ++k) {
+  // columns four at a time, so that 4 * kRows loads are in flight
+  int k = 0;
+  for (; k + 4 <= j; k += 4) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int i = base + r * kBlock;
+      if (i < p.n) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] += wv[r] * V[(k + q) * n + i];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double t = op2hip::wave_sum(a[q]);
+      if (lane == 0) dsh[(k + q) * kWaves + wave] = t;
+    }
+  }
+  for (; k < j; ++k) {
     const double* __restrict__ vk = V + k * n;
     double a = 0.0;
 #pragma unroll
@@ -284,10 +301,10 @@ __global__ void __launch_bounds__(kBlock) k_g_dots(Prob p) {
     if (lane == 0) dsh[j * kWaves + wave] = a;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k <= j; k += kBlock) {
+  for (int c = threadIdx.x; c <= j; c += kBlock) {
     double t = 0.0;
-    for (int q = 0; q < kWaves; ++q) t += dsh[k * kWaves + q];
-    p.part[(size_t)blockIdx.x * p.ps + k] = t;
+    for (int q = 0; q < kWaves; ++q) t += dsh[c * kWaves + q];
+    p.part[(size_t)c * p.ps + blockIdx.x] = t;
   }
 }
 
@@ -300,7 +317,8 @@ __global__ void __launch_bounds__(kRBlock) k_g_coef(Prob p) {
   const int lane = threadIdx.x & 63;
   for (int k = threadIdx.x >> 6; k <= j; k += kRBlock / 64) {
     double v = 0.0;
-    for (int b = lane; b < p.nb; b += 64) v += p.part[(size_t)b * p.ps + k];
+    const double* __restrict__ pk = p.part + (size_t)k * p.ps;
+    for (int b = lane; b < p.nb; b += 64) v += pk[b];
     v = op2hip::wave_sum(v);
     if (lane == 0) {
       p.lhh[k] = -v;
@@ -326,7 +344,25 @@ __global__ void __launch_bounds__(kBlock) k_g_orth(Prob p) {
     const int i = base + r * kBlock;
     wv[r] = i < p.n ? w[i] : 0.0;
   }
-  for (int k = 0; k <= j; ++k) {
+  // columns four at a time (loads issued together, applied in column order)
+  int k = 0;
+  for (; k + 4 <= j + 1; k += 4) {
+    double v[4][kRows];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int i = base + r * kBlock;
+        v[q][r] = i < p.n ? V[(k + q) * n + i] : 0.0;
+      }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double c = lhh[k + q];
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) wv[r] = wv[r] + c * v[q][r];
+    }
+  }
+  for (; k <= j; ++k) {
     const double c = lhh[k];
     const double* __restrict__ vk = V + k * n;
 #pragma unroll
@@ -657,7 +693,8 @@ int spmv_blocks(int n) {
 // vectors and (GMRES) the small dense arrays
 size_t layout(int method, int n, int m, Prob* p, double* w) {
   const int nb = tiles(n), nbs = spmv_blocks(n);
-  const int ps = m + 1 > 3 ? m + 1 : 3;
+  const int ps = nb > nbs ? nb : nbs;
+  const int nq = m + 1 > 3 ? m + 1 : 3;  // quantities reduced at once
   size_t off = 0;
   auto take = [&](size_t len) {
     double* q = w ? w + off : nullptr;
@@ -665,7 +702,7 @@ size_t layout(int method, int n, int m, Prob* p, double* w) {
     return q;
   };
   State* st = reinterpret_cast<State*>(take(32));
-  double* part = take((size_t)(nb > nbs ? nb : nbs) * ps);
+  double* part = take((size_t)nq * ps);
   double* inv = take(n);
   if (p) {
     p->st = st;
