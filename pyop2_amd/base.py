@@ -198,6 +198,20 @@ class Set(object):
     IMPORT_EXEC_SIZE = 2
     IMPORT_NON_EXEC_SIZE = 3
 
+    @classmethod
+    def fromhdf5(cls, f, name, dim=1):
+        """The Set named ``name`` in HDF5 data ``f`` (an h5py-like mapping,
+        e.g. ``pyop2_amd.hdf5.File``): a one-element dataset holding the
+        size, with an optional ``dim`` attribute (reference:
+        base.py:386-394)."""
+        slot = f[name]
+        if tuple(slot.shape) != (1,):
+            raise SizeTypeError("Shape of %s is incorrect" % name)
+        size = np.asarray(slot[()]).astype(np.int64)
+        dim = slot.attrs.get('dim', dim)
+        dim = tuple(int(d) for d in np.atleast_1d(dim)) if np.ndim(dim) else int(dim)
+        return cls(int(size[0]), dim, name)
+
     @validate_type(('size', (int, tuple, list), SizeTypeError),
                    ('name', str, NameTypeError))
     def __init__(self, size=None, dim=1, name=None, halo=None):
@@ -397,6 +411,17 @@ class Dat(DataCarrier):
     _globalcount = 0
     _modes = [READ, WRITE, RW, INC]
 
+    @classmethod
+    def fromhdf5(cls, dataset, f, name):
+        """The Dat named ``name`` in HDF5 data ``f`` on ``dataset``; its
+        ``type`` attribute marks structure-of-arrays data with ':soa'
+        (reference: base.py:842-849)."""
+        slot = f[name]
+        t = slot.attrs['type']
+        if isinstance(t, bytes):
+            t = t.decode('utf-8')
+        return cls(dataset, np.asarray(slot[()]), name=name, soa=str(t).find(':soa') > 0)
+
     @validate_type(('dataset', Set, SetTypeError), ('name', str, NameTypeError))
     def __init__(self, dataset, data=None, dtype=None, name=None, soa=None, uid=None):
         if data is None:
@@ -580,6 +605,16 @@ class Const(DataCarrier):
     _sorted = (-1, [])
     _globalcount = 0
 
+    @classmethod
+    def fromhdf5(cls, f, name):
+        """The Const named ``name`` in HDF5 data ``f``; its dim is the
+        dataset's shape (reference: base.py:918-926)."""
+        slot = f[name]
+        dim = tuple(slot.shape)
+        if len(dim) < 1:
+            raise DimTypeError("Invalid dimension value %s" % (dim,))
+        return cls(dim, np.asarray(slot[()]), name)
+
     @validate_type(('name', str, NameTypeError))
     def __init__(self, dim, data=None, name=None, dtype=None):
         self._dim = as_tuple(dim, int)
@@ -716,6 +751,17 @@ class Map(object):
     (reference: base.py:1018-1125)."""
 
     _globalcount = 0
+
+    @classmethod
+    def fromhdf5(cls, iterset, dataset, f, name):
+        """The Map named ``name`` in HDF5 data ``f``: a two-dimensional
+        dataset whose second extent is the map's dim (reference:
+        base.py:1117-1125)."""
+        slot = f[name]
+        dim = tuple(slot.shape)[1:]
+        if len(dim) != 1:
+            raise DimTypeError("Unrecognised dimension value %s" % (dim,))
+        return cls(iterset, dataset, int(dim[0]), np.asarray(slot[()]), name)
 
     @validate_type(('iterset', Set, SetTypeError), ('dataset', Set, SetTypeError),
                    ('dim', int, DimTypeError), ('name', str, NameTypeError))

@@ -238,6 +238,44 @@ class Airfoil(object):
             for k, v in CONSTS.items():
                 self.consts.append(W.Const(1, v, k, np.float64))
             self.consts.append(W.Const(4, free_stream(), 'qinf', np.float64))
+        self._kernels()
+
+    CONST_NAMES = ('gam', 'gm1', 'cfl', 'eps', 'mach', 'alpha', 'qinf')
+
+    @classmethod
+    def fromhdf5(cls, W, f, consts=True):
+        """The problem read from HDF5 data ``f`` (an h5py-like mapping such
+        as ``pyop2_amd.hdf5.File``) the way the reference's demo reads
+        ``new_grid.h5`` (demo/airfoil.py:45-76): Sets, Maps, Dats and Consts
+        through their ``fromhdf5`` constructors."""
+        self = cls.__new__(cls)
+        self.W = W
+        self.mesh = None
+        self.nodes = W.Set.fromhdf5(f, 'nodes', dim=2)
+        self.edges = W.Set.fromhdf5(f, 'edges')
+        self.bedges = W.Set.fromhdf5(f, 'bedges')
+        self.cells = W.Set.fromhdf5(f, 'cells')
+        self.vcells = W.Set.fromhdf5(f, 'cells', dim=4)
+        self.pedge = W.Map.fromhdf5(self.edges, self.nodes, f, 'pedge')
+        self.pecell = W.Map.fromhdf5(self.edges, self.cells, f, 'pecell')
+        self.pevcell = W.Map.fromhdf5(self.edges, self.vcells, f, 'pecell')
+        self.pbedge = W.Map.fromhdf5(self.bedges, self.nodes, f, 'pbedge')
+        self.pbecell = W.Map.fromhdf5(self.bedges, self.cells, f, 'pbecell')
+        self.pbevcell = W.Map.fromhdf5(self.bedges, self.vcells, f, 'pbecell')
+        self.pcell = W.Map.fromhdf5(self.cells, self.nodes, f, 'pcell')
+        self.p_bound = W.Dat.fromhdf5(self.bedges, f, 'p_bound')
+        self.p_x = W.Dat.fromhdf5(self.nodes, f, 'p_x')
+        self.p_q = W.Dat.fromhdf5(self.vcells, f, 'p_q')
+        self.p_qold = W.Dat.fromhdf5(self.vcells, f, 'p_qold')
+        self.p_adt = W.Dat.fromhdf5(self.cells, f, 'p_adt')
+        self.p_res = W.Dat.fromhdf5(self.vcells, f, 'p_res')
+        self.rms = W.Global(1, 0.0, np.float64, 'rms')
+        self.consts = [W.Const.fromhdf5(f, k) for k in cls.CONST_NAMES] if consts else []
+        self._kernels()
+        return self
+
+    def _kernels(self):
+        W = self.W
         self.k_save_soln = W.Kernel(SAVE_SOLN, 'save_soln')
         self.k_adt_calc = W.Kernel(ADT_CALC, 'adt_calc')
         self.k_res_calc = W.Kernel(RES_CALC, 'res_calc')
@@ -298,8 +336,9 @@ class AirfoilVector(Airfoil):
     edge's vector maps, bres_calc reads its nodes through one.  Results
     equal :class:`Airfoil`'s bit for bit."""
 
-    def __init__(self, W, mesh, q0=None, consts=True):
-        super(AirfoilVector, self).__init__(W, mesh, q0, consts)
+    def _kernels(self):
+        super(AirfoilVector, self)._kernels()
+        W = self.W
         self.k_adt_calc = W.Kernel(ADT_CALC_VEC, 'adt_calc')
         self.k_res_calc = W.Kernel(RES_CALC_VEC, 'res_calc')
         self.k_bres_calc = W.Kernel(BRES_CALC_VEC, 'bres_calc')

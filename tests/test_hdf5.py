@@ -1,0 +1,182 @@
+"""HDF5 mesh ingestion: Set/Map/Dat/Const.fromhdf5 (reference
+pyop2/base.py:386-394, 842-849, 918-926, 1117-1125) and the airfoil demo's
+reading of new_grid.h5 (demo/airfoil.py:45-76), through pyop2_amd.hdf5.
+
+The reader is pinned by files the HDF5 library wrote, with the contents
+their own test suite states: PyTables' copies of the NCSA sample files
+(tables/tests/test_hdf5compat.py: /TestArray is the 6 x 5 array i + j in
+int32, int64 and float64, big- and little-endian; /ExtendibleArray a
+chunked, extendible 10 x 5 int32), read where this machine has them.  The
+committed fixture tests/golden/airfoil_24x12.h5 (written by
+tests/golden/make_airfoil_h5.py with tests/golden/h5write.py) must read
+back as the arrays it was made from, and an airfoil built from it through
+fromhdf5 must run exactly as one built from the same arrays directly (CPU
+oracle) and as the oracle on the GPU (bit-exact q and res)."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import sequential as seq
+from pyop2_amd import hdf5
+from pyop2_amd.exceptions import DimTypeError, SizeTypeError
+from pyop2_amd.workloads import airfoil, meshes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, 'golden'))
+
+import h5write  # noqa: E402
+import make_airfoil_h5  # noqa: E402
+
+SAMPLES = '/opt/conda/lib/python3.9/site-packages/tables/tests'
+needs_samples = pytest.mark.skipif(not os.path.isdir(SAMPLES),
+                                   reason='HDF5 sample files not on this machine')
+
+
+@needs_samples
+@pytest.mark.parametrize('name,dtype', [('smpl_i32be', np.int32), ('smpl_i32le', np.int32),
+                                        ('smpl_i64be', np.int64), ('smpl_i64le', np.int64),
+                                        ('smpl_f64be', np.float64), ('smpl_f64le', np.float64)])
+def test_reader_library_samples(name, dtype):
+    with hdf5.File(os.path.join(SAMPLES, name + '.h5')) as f:
+        assert f.keys() == ['TestArray']
+        ds = f['/TestArray']
+        assert ds.shape == (6, 5) and ds.dtype == np.dtype(dtype)
+        v = ds[()]
+        assert v.dtype == np.dtype(dtype)
+        assert np.array_equal(v, np.add.outer(np.arange(6), np.arange(5)))
+
+
+@needs_samples
+def test_reader_chunked_extendible():
+    with hdf5.File(os.path.join(SAMPLES, 'smpl_SDSextendible.h5')) as f:
+        v = f['ExtendibleArray'].value
+    expect = np.zeros((10, 5), dtype=np.int32)
+    expect[:2, :3] = 1
+    expect[2, :3] = 1
+    expect[:2, 3:] = 3
+    expect[3:, 0] = 2
+    assert v.dtype == np.int32 and np.array_equal(v, expect)
+
+
+def _read_all(g):
+    for k in g.keys():
+        o = g[k]
+        if isinstance(o, hdf5.Group):
+            _read_all(o)
+        else:
+            o[()]
+
+
+@needs_samples
+def test_reader_unsupported_is_explicit():
+    """A compound (table) dataset is refused with NotImplementedError."""
+    with hdf5.File(os.path.join(SAMPLES, 'non-chunked-table.h5')) as f:
+        with pytest.raises(NotImplementedError):
+            _read_all(f)
+
+
+def test_not_hdf5(tmp_path):
+    p = tmp_path / 'x.h5'
+    p.write_bytes(b'not an hdf5 file' * 64)
+    with pytest.raises(ValueError):
+        hdf5.File(str(p))
+    with pytest.raises(ValueError):
+        hdf5.File(str(p), 'w')
+
+
+def test_fixture_reads_back():
+    mesh, q, data = make_airfoil_h5.datasets()
+    with hdf5.File(make_airfoil_h5.PATH) as f:
+        assert f.keys() == sorted(data)
+        for k, (a, attrs) in data.items():
+            ds = f[k]
+            assert ds.shape == a.shape, k
+            assert np.array_equal(ds[()], a), k
+            for an, av in attrs.items():
+                assert ds.attrs[an] == av
+
+
+def test_writer_types_and_attributes(tmp_path):
+    p = str(tmp_path / 't.h5')
+    a = np.arange(12, dtype=np.float32).reshape(3, 4)
+    b = -np.arange(5, dtype=np.int64)
+    h5write.write(p, {'a': (a, {'type': 'float:soa', 'dim': np.int32(4)}),
+                      'b': (b, {'scale': np.array([0.5, 2.0])}),
+                      'empty': (np.zeros((0, 2)), {})})
+    with hdf5.File(p) as f:
+        assert np.array_equal(f['a'][()], a) and f['a'].dtype == np.float32
+        assert f['a'].attrs['type'] == 'float:soa' and f['a'].attrs['dim'] == 4
+        assert np.array_equal(f['b'][()], b)
+        assert np.array_equal(f['b'].attrs['scale'], [0.5, 2.0])
+        assert f['empty'].shape == (0, 2) and f['empty'][()].shape == (0, 2)
+        with pytest.raises(KeyError):
+            f['missing']
+
+
+def test_fromhdf5_constructors(tmp_path):
+    """The reference's checks: a Set needs a one-element dataset, a Map two
+    dimensions, a Const at least one; 'dim' attributes and ':soa' types
+    are honoured."""
+    p = str(tmp_path / 'c.h5')
+    h5write.write(p, {'s': (np.array([7], np.int32), {'dim': np.array([2, 3], np.int32)}),
+                      'bad': (np.array([7, 8], np.int32), {}),
+                      'm': (np.arange(14, dtype=np.int32).reshape(7, 2) % 7, {}),
+                      'm1': (np.arange(7, dtype=np.int32), {}),
+                      'd': (np.ones((7, 6)), {'type': 'double:soa'}),
+                      'k': (np.array([1.5, 2.5]), {})})
+    with hdf5.File(p) as f:
+        s = seq.Set.fromhdf5(f, 's')
+        assert s.size == 7 and s.dim == (2, 3)
+        with pytest.raises(SizeTypeError):
+            seq.Set.fromhdf5(f, 'bad')
+        t = seq.Set.fromhdf5(f, 's', dim=5)
+        assert t.dim == (2, 3)           # the attribute wins over the argument
+        m = seq.Map.fromhdf5(s, s, f, 'm')
+        assert m.dim == 2 and np.array_equal(m.values, f['m'][()])
+        with pytest.raises(DimTypeError):
+            seq.Map.fromhdf5(s, s, f, 'm1')
+        d = seq.Dat.fromhdf5(s, f, 'd')
+        assert d.soa and np.array_equal(d.data_ro.reshape(7, 6), np.ones((7, 6)))
+        k = seq.Const.fromhdf5(f, 'k')
+        try:
+            assert k.dim == (2,) and np.array_equal(k.data, [1.5, 2.5])
+        finally:
+            k.remove_from_namespace()
+
+
+def _run(A, iters=2):
+    for _ in range(iters):
+        A.iteration()
+    return (np.array(A.p_q.data_ro), np.array(A.p_res.data_ro), np.array(A.p_adt.data_ro),
+            float(A.rms.data[0]))
+
+
+def test_airfoil_fromhdf5_matches_direct():
+    """The demo's reading order yields the same problem as the arrays it was
+    written from: two iterations agree bit for bit on the oracle."""
+    mesh, q, _ = make_airfoil_h5.datasets()
+    with hdf5.File(make_airfoil_h5.PATH) as f:
+        A = airfoil.Airfoil.fromhdf5(seq, f)
+    try:
+        B = airfoil.Airfoil(seq, mesh, q0=q, consts=False)
+        for a, b in zip(_run(A), _run(B)):
+            assert np.array_equal(a, b)
+    finally:
+        A.remove_consts()
+
+
+@pytest.mark.gpu
+def test_airfoil_fromhdf5_hip(hip):
+    with hdf5.File(make_airfoil_h5.PATH) as f:
+        A = airfoil.Airfoil.fromhdf5(hip, f)
+        B = airfoil.Airfoil.fromhdf5(seq, f, consts=False)
+    try:
+        qa, ra, da, rmsa = _run(A, 3)
+        qb, rb, db, rmsb = _run(B, 3)
+        assert np.array_equal(qa, qb) and np.array_equal(ra, rb) and np.array_equal(da, db)
+        assert abs(rmsa - rmsb) <= 1e-12 * abs(rmsb)
+    finally:
+        A.remove_consts()
