@@ -180,3 +180,55 @@ def test_airfoil_fromhdf5_hip(hip):
         assert abs(rmsa - rmsb) <= 1e-12 * abs(rmsb)
     finally:
         A.remove_consts()
+
+
+class TestHDF5Backends:
+    """The reference's test/unit/test_hdf5.py on every backend, its file
+    made with the test writer instead of h5py (same datasets and
+    attributes)."""
+
+    @pytest.fixture(scope='class')
+    def h5file(self, tmp_path_factory):
+        p = str(tmp_path_factory.mktemp('h5') / 'tmp_hdf5.h5')
+        h5write.write(p, {
+            'dat': (np.arange(10, dtype=np.float64).reshape(5, 2), {'type': 'double'}),
+            'soadat': (np.arange(10, dtype=np.float64).reshape(5, 2), {'type': 'double:soa'}),
+            'set': (np.array((5,)), {'dim': 2}),
+            'myconstant': (np.arange(3), {}),
+            'map': (np.array((1, 2, 2, 3)).reshape(2, 2), {})})
+        f = hdf5.File(p)
+        yield f
+        f.close()
+
+    @pytest.fixture
+    def op2(self, backend):
+        from pyop2_amd import op2
+        return op2
+
+    def test_set_hdf5(self, op2, h5file):
+        s = op2.Set.fromhdf5(h5file, name='set')
+        assert s.size == 5 and s.dim == (2,)
+
+    def test_dat_hdf5(self, op2, h5file):
+        d = op2.Dat.fromhdf5(op2.Set(5, 2, 'foo'), h5file, 'dat')
+        assert d.dtype == np.float64
+        assert d.data.shape == (5, 2) and d.data.sum() == 9 * 10 / 2
+
+    def test_data_hdf5_soa(self, op2, h5file):
+        d = op2.Dat.fromhdf5(op2.Set(5, 2, 'foo'), h5file, 'soadat')
+        assert d.soa
+        assert d.data.shape == (5, 2) and d.data.sum() == 9 * 10 / 2
+
+    def test_const_hdf5(self, op2, h5file):
+        c = op2.Const.fromhdf5(h5file, 'myconstant')
+        c.remove_from_namespace()
+        assert c.data.sum() == 3
+        assert c.dim == (3,)
+
+    def test_map_hdf5(self, op2, h5file):
+        iterset, dataset = op2.Set(2, 1, 'iterset'), op2.Set(3, 1, 'dataset')
+        m = op2.Map.fromhdf5(iterset, dataset, h5file, name="map")
+        assert m.iterset == iterset and m.dataset == dataset
+        assert m.dim == 2
+        assert m.values.sum() == sum((1, 2, 2, 3))
+        assert m.name == 'map'
