@@ -85,6 +85,9 @@ def parse():
                    help='requests per arrival rate in the serving measurement (0: skip)')
     p.add_argument('--serve-pool', type=int, default=4,
                    help='2-D and 3-D meshes each in the serving pool')
+    p.add_argument('--solvers', type=float, default=2e6,
+                   help='elements of the 2-D system the CG, GMRES(30) and BiCGStab '
+                        'iterations are timed on (0: skip)')
     p.add_argument('--copy-calibration', action='store_true',
                    help='only run a 256 MiB device copy three times (the reference reading '
                         'for rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit')
@@ -290,6 +293,9 @@ def main():
         pool = suites.make_pool(op2, n2d=args.serve_pool, n3d=args.serve_pool,
                                 tri_range=(8000, 100000), tet_range=(50000, 200000))
         extra['serving'] = suites.serve_suite(op2, pool, nreq=args.serve)
+    if args.solvers:
+        from pyop2_amd.workloads import cgbench
+        extra['solvers'] = cgbench.run(op2, int(args.solvers), 200)
     if extra:
         out['extra'] = extra
     print(json.dumps(out))

@@ -23,15 +23,14 @@ import sys
 import time
 
 
-def main(argv):
+def run(op2, ne=2000000, its=300):
+    """Time the three solvers on one assembled system (``op2`` initialised
+    with the hip backend); returns one result dict per method."""
     import torch
-    from pyop2_amd import op2, batch
+    from pyop2_amd import batch
     from pyop2_amd.workloads import suites
-    ne = int(float(argv[0])) if argv else 2000000
-    its = int(argv[1]) if len(argv) > 1 else 300
-    op2.init(backend='hip')
     mesh = suites.tri_mesh_of(ne, 1)
-    B = batch.MeshBatch(op2, [mesh])
+    B = batch.MeshBatch(op2, [mesh], name='cgbench')
     B.assemble()
     B.solve(rtol=1e-10, maxit=20)
     torch.cuda.synchronize()
@@ -40,7 +39,7 @@ def main(argv):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t
     n, nnz, done = mesh.nvert, B.sparsity.nz, r[0][0]
-    report('cg', mesh, n, nnz, done, dt, 12 * nnz + 108 * n)
+    out = [report('cg', mesh, n, nnz, done, dt, 12 * nnz + 108 * n)]
     m = 30
     gm = 12 * nnz + 28 * n + sum(8 * n * (j + 2) + 8 * n * (j + 3) for j in range(m)) / m
     for method, per_it in (('gmres', gm), ('bcgs', 24 * nnz + 168 * n)):
@@ -54,14 +53,24 @@ def main(argv):
         solver.solve(B.A, B.x, B.b)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
-        report(method, mesh, n, nnz, solver.getIterationNumber(), dt, per_it)
+        out.append(report(method, mesh, n, nnz, solver.getIterationNumber(), dt, per_it))
+    return out
 
 
 def report(method, mesh, n, nnz, done, dt, per_it):
-    print(json.dumps({'method': method, 'elements': mesh.nelem, 'rows': n, 'nnz': nnz,
-                      'iterations': done, 'us_per_iteration': round(dt / done * 1e6, 2),
-                      'bytes_per_iteration': int(per_it),
-                      'GBps': round(per_it * done / dt / 1e9, 1)}), flush=True)
+    return {'method': method, 'elements': mesh.nelem, 'rows': n, 'nnz': nnz,
+            'iterations': done, 'us_per_iteration': round(dt / done * 1e6, 2),
+            'bytes_per_iteration': int(per_it),
+            'GBps': round(per_it * done / dt / 1e9, 1)}
+
+
+def main(argv):
+    from pyop2_amd import op2
+    ne = int(float(argv[0])) if argv else 2000000
+    its = int(argv[1]) if len(argv) > 1 else 300
+    op2.init(backend='hip')
+    for r in run(op2, ne, its):
+        print(json.dumps(r), flush=True)
 
 
 if __name__ == '__main__':
