@@ -128,7 +128,7 @@ int oracle_pcg(int n, const int* rp, const int* ci, const double* vals, const do
   double rnorm0 = dp;
   double ttol = std::fmax(rtol * rnorm0, atol);
   auto test = [&](double d) -> int {
-    if (d != d) return -9;
+    if (d != d || std::isinf(d)) return -9;
     if (d <= ttol) return d < atol ? 3 : 2;
     if (d >= dtol * rnorm0) return -4;
     return 0;
@@ -183,7 +183,7 @@ static int ksp_test(int k, double rn, double rtol, double atol, double dtol, dou
     rnorm0 = rn;
     ttol = std::fmax(rtol * rnorm0, atol);
   }
-  if (rn != rn) return -9;
+  if (rn != rn || std::isinf(rn)) return -9;
   if (rn <= ttol) return rn < atol ? 3 : 2;
   if (rn >= dtol * rnorm0) return -4;
   return 0;

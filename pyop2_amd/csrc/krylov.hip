@@ -96,7 +96,7 @@ __device__ inline int ksp_test(State& s, double rn) {
     s.rnorm0 = rn;
     s.ttol = fmax(s.rtol * rn, s.atol);
   }
-  if (!(rn == rn)) return KSP_DIVERGED_NANORINF;
+  if (!(rn == rn) || isinf(rn)) return KSP_DIVERGED_NANORINF;
   if (rn <= s.ttol) return rn < s.atol ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
   if (rn >= s.dtol * s.rnorm0) return KSP_DIVERGED_DTOL;
   return 0;
@@ -107,8 +107,8 @@ __device__ inline void log_res(const Prob& p, int k, double v) {
 }
 
 // sums of partials k0 .. k0+K-1 over the nparts workgroups of a row
-// kernel, in one pass (one workgroup of kRBlock; sh holds K * kRBlock / 64 doubles); the
-// results are valid in thread 0
+// kernel, in one pass (one workgroup of kRBlock; sh holds K * kRBlock / 64
+// doubles); the results are valid in thread 0
 template <int K>
 __device__ inline void reduce_parts(const Prob& p, int nparts, int k0, double* out, double* sh) {
   double v[K];
@@ -150,8 +150,8 @@ __device__ inline void store_part(const Prob& p, int k, double v, double* sh) {
 #define ROWS(i) for (int i = blockIdx.x * kBlock + threadIdx.x; i < p.n; i += p.nb * kBlock)
 
 // SpMV rows: `lanes` threads per row, grid-stride over row groups of the
-// nbs workgroups.  F(row,
-// sum) runs on the row's first lane; G(col) is the gathered operand.
+// nbs workgroups.  finish(row, sum) runs on the row's first lane;
+// gather(col) is the gathered operand.
 template <class G, class F>
 __device__ inline void spmv_rows(const Prob& p, G gather, F finish) {
   const int lanes = p.lanes;
