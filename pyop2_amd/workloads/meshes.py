@@ -158,3 +158,45 @@ class TetMesh(object):
         self.coords = coords
         self.elem_node = elem_node.astype(np.int32)
         self.nelem, self.nvert = el.shape[0], nv
+
+
+def _triangle_table(path):
+    """Rows of a Triangle .node/.ele file after its header: comments ('#' to
+    end of line) and blank lines dropped."""
+    with open(path) as h:
+        lines = [ln.split('#', 1)[0].split() for ln in h]
+    lines = [ln for ln in lines if ln]
+    return [int(t) for t in lines[0][:3]], lines[1:]
+
+
+def read_triangle(op2, prefix):
+    """Read the Triangle mesh ``prefix``.node / ``prefix``.ele into op2
+    objects, as the reference's demos do (demo/triangle_reader.py:39-89):
+    1-based ids, x and y of each node, the first three nodes of each
+    triangle; attributes and boundary markers ignored.  Returns
+    ``(nodes, vnodes, coords, elements, elem_node, elem_vnode)``."""
+    head, rows = _triangle_table(prefix + '.node')
+    num_nodes = head[0]
+    if len(rows) != num_nodes:
+        raise ValueError("%s.node: header says %d nodes, %d given" % (prefix, num_nodes, len(rows)))
+    ids = np.array([int(r[0]) for r in rows]) - 1
+    xy = np.array([[float(r[1]), float(r[2])] for r in rows])
+    coord_values = np.zeros((num_nodes, 2))
+    coord_values[ids] = xy
+    nodes = op2.Set(num_nodes, 1, "nodes")
+    vnodes = op2.Set(num_nodes, 2, "vnodes")
+    coords = op2.Dat(vnodes, coord_values, np.float64, "coords")
+
+    (num_tri, nodes_per_tri, _), rows = _triangle_table(prefix + '.ele')
+    if nodes_per_tri != 3:
+        raise ValueError("%s.ele: %d nodes per triangle; linear triangles (3) only"
+                         % (prefix, nodes_per_tri))
+    if len(rows) != num_tri:
+        raise ValueError("%s.ele: header says %d triangles, %d given" % (prefix, num_tri, len(rows)))
+    tri = np.array([int(r[0]) for r in rows]) - 1
+    map_values = np.zeros((num_tri, 3), dtype=np.int32)
+    map_values[tri] = np.array([[int(v) - 1 for v in r[1:4]] for r in rows])
+    elements = op2.Set(num_tri, 1, "elements")
+    elem_node = op2.Map(elements, nodes, 3, map_values, "elem_node")
+    elem_vnode = op2.Map(elements, vnodes, 3, map_values, "elem_vnode")
+    return nodes, vnodes, coords, elements, elem_node, elem_vnode
