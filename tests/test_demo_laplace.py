@@ -1,11 +1,13 @@
-"""The reference's Laplace demo (demo/laplace_ffc.py:90-160) as a whole
-program: P1 Laplacian on 8 triangles of the unit square, zero source,
-strong Dirichlet conditions u = 1 on y = 0 and u = 2 on y = 1 by
-Mat.zero_rows on those rows and a WRITE loop into the right-hand side
-through the boundary-node map, then Solver(linear_solver='gmres') with the
-reference's defaults.  The demo states the answer: u = 1 + y at the nine
-nodes (the discrete solution is exact, u being linear).  The FFC-generated
-kernels are replaced by hand-written P1 kernels (coefficient 1)."""
+"""The reference's Laplace demos (demo/laplace_ffc.py:90-160,
+weak_bcs_ffc.py:90-185) as whole programs.  laplace_ffc: P1 Laplacian on 8
+triangles of the unit square, zero source, strong Dirichlet conditions
+u = 1 on y = 0 and u = 2 on y = 1 by Mat.zero_rows on those rows and a
+WRITE loop into the right-hand side through the boundary-node map, then
+Solver(linear_solver='gmres') with the reference's defaults; the demo
+states the answer u = 1 + y at the nine nodes (exact, u being linear).
+weak_bcs_ffc replaces the top condition by a flux through a facet loop.
+The FFC-generated kernels are replaced by hand-written P1 kernels
+(coefficient 1)."""
 
 import numpy as np
 import pytest
@@ -71,3 +73,73 @@ def test_laplace_demo_cg_fails(backend):
     error_on_nonconvergence raises."""
     with pytest.raises(RuntimeError):
         laplace_demo('cg')
+
+
+# The weak_bcs demo's boundary term v g ds on the facet given by a Global:
+# facet k of a triangle is the edge opposite its vertex k (UFC numbering);
+# exact for P1 g: each end point gets |e|/6 (2 g_own + g_other).
+WEAK = """
+void weak(double A[1], double *x[2], double **f, double **g, unsigned int *facet, int j)
+{
+  int a = (*facet + 1) % 3, b = (*facet + 2) % 3;
+  double dx = x[b][0] - x[a][0], dy = x[b][1] - x[a][1];
+  double len = sqrt(dx * dx + dy * dy);
+  if (j == a) A[0] += len / 6.0 * (2.0 * g[a][0] + g[b][0]);
+  if (j == b) A[0] += len / 6.0 * (2.0 * g[b][0] + g[a][0]);
+}
+"""
+
+# the demo's cell right-hand side takes the gradient Dat and ignores it
+RHS_IGNORING = """
+void rhs(double A[1], double *x[2], double **f, double **ignored, int j)
+{
+  rhs_tri(A, x, f, j);
+}
+"""
+
+
+@pytest.mark.parametrize('method', ['gmres', 'bcgs'])
+def test_weak_bcs_demo(backend, method):
+    """demo/weak_bcs_ffc.py:90-185: u = 1 on y = 0 (strong), du/dn = 2 on
+    y = 1 (weak, through a facet loop over the two top elements with the
+    facet number in a uint32 Global), zero source; the demo states
+    u = 1 + 2y at the nodes."""
+    nodes = op2.Set(NUM_NODES, 1, "nodes")
+    vnodes = op2.Set(NUM_NODES, 2, "vnodes")
+    elements = op2.Set(NUM_ELE, 1, "elements")
+    top = op2.Set(2, 1, "top_boundary_elements")
+    bdry_nodes = op2.Set(3, 1, "boundary_nodes")
+    elem_node = op2.Map(elements, nodes, 3, ELEM_NODE, "elem_node")
+    elem_vnode = op2.Map(elements, vnodes, 3, ELEM_NODE, "elem_vnode")
+    top_map = np.asarray([7, 6, 3, 8, 7, 4], dtype=np.float64)
+    top_node = op2.Map(top, nodes, 3, top_map, "top_bdry_elem_node")
+    top_vnode = op2.Map(top, vnodes, 3, top_map, "top_bdry_elem_vnode")
+    bdry_node_node = op2.Map(bdry_nodes, nodes, 1, np.asarray([0, 1, 2], dtype=np.float64),
+                             "bdry_node_node")
+    mat = op2.Mat(op2.Sparsity((elem_node, elem_node), "sparsity"), np.float64, "mat")
+    coords = op2.Dat(vnodes, COORDS.copy(), np.float64, "coords")
+    kappa = op2.Dat(nodes, np.ones(NUM_NODES), np.float64, "kappa")
+    f = op2.Dat(nodes, np.zeros(NUM_NODES), np.float64, "f")
+    b = op2.Dat(nodes, np.zeros(NUM_NODES), np.float64, "b")
+    x = op2.Dat(nodes, np.zeros(NUM_NODES), np.float64, "x")
+    bdry = op2.Dat(bdry_nodes, np.ones(3), np.float64, "bdry")
+    bdry_grad = op2.Dat(nodes, np.full(NUM_NODES, 2.0), np.float64, "gradient")
+    facet = op2.Global(1, 2, np.uint32, "facet")
+
+    op2.par_loop(op2.Kernel(fem.LAPLACE_TRI, "laplace_tri"), elements(3, 3),
+                 mat((elem_node[op2.i[0]], elem_node[op2.i[1]]), op2.INC),
+                 coords(elem_vnode, op2.READ), kappa(elem_node, op2.READ))
+    op2.par_loop(op2.Kernel(fem.RHS_TRI + RHS_IGNORING, "rhs"), elements(3),
+                 b(elem_node[op2.i[0]], op2.INC), coords(elem_vnode, op2.READ),
+                 f(elem_node, op2.READ), bdry_grad(elem_node, op2.READ))
+    op2.par_loop(op2.Kernel(WEAK, "weak"), top(3),
+                 b(top_node[op2.i[0]], op2.INC), coords(top_vnode, op2.READ),
+                 f(top_node, op2.READ), bdry_grad(top_node, op2.READ), facet(op2.READ))
+    mat.zero_rows([0, 1, 2], 1.0)
+    op2.par_loop(op2.Kernel(STRONGBC_RHS, "strongbc_rhs"), bdry_nodes,
+                 bdry(op2.IdentityMap, op2.READ), b(bdry_node_node[0], op2.WRITE))
+    solver = op2.Solver(linear_solver=method)
+    solver.solve(mat, x, b)
+    assert solver.getConvergedReason() > 0
+    expected = np.asarray([1.0, 1.0, 1.0, 2.0, 2.0, 2.0, 3.0, 3.0, 3.0])
+    assert np.abs(x.data_ro.ravel() - expected).max() < 1e-6
