@@ -37,6 +37,7 @@ constexpr int kMaxSpmvBlocks = 2048;
 constexpr int kChunk = 32;       // most GMRES steps per round (a cycle, up to 32)
 constexpr int kBcgsChunk = 16;   // BiCGStab iterations per round
 constexpr int kMaxRestart = 1024;  // LDS of k_g_dots and k_g_rot
+constexpr size_t kSolveLds = 60 * 1024;  // k_g_solve stages H in LDS up to this
 constexpr double kHapTol = 1e-30;
 
 enum { METHOD_GMRES = 1, METHOD_BCGS = 2 };
@@ -450,33 +451,52 @@ __global__ void __launch_bounds__(kRBlock) k_g_rot(Prob p) {
 }
 
 // cycle end: y from the triangular system in grs (KSPGMRESBuildSoln), then
-// DONE or RESTART
-__global__ void k_g_solve(Prob p) {
+// DONE or RESTART.  The triangle and the right-hand side are staged in LDS
+// for the serial back substitution when (m + 1)^2 doubles fit in
+// kSolveLds (the launch then gives that much dynamic LDS).
+__device__ inline bool solve_staged(int m) {
+  return sizeof(double) * (size_t)(m + 1) * (m + 1) <= kSolveLds;
+}
+
+__global__ void __launch_bounds__(kBlock) k_g_solve(Prob p) {
+  extern __shared__ double lds[];
   State& s = *p.st;
-  if (s.phase != PH_END || threadIdx.x != 0) return;
+  if (s.phase != PH_END) return;
   const int top = s.it - 1;
-  const size_t ld = p.m + 1;
-  double* g = p.grs;
+  const int ld = p.m + 1;
+  const bool staged = solve_staged(p.m);
+  const double* tri = staged ? lds : p.H;
+  double* g = staged ? lds + (size_t)ld * p.m : p.grs;  // after the m columns of H
+  if (staged) {
+    for (int k = threadIdx.x; k < (top + 1) * ld; k += kBlock) lds[k] = p.H[k];
+    for (int k = threadIdx.x; k <= top; k += kBlock) g[k] = p.grs[k];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   s.upd = 0;
   if (top >= 0) {
     bool ok = true;
-    if (p.H[top * ld + top] != 0.0) {
-      g[top] = g[top] / p.H[top * ld + top];
+    if (tri[top * ld + top] != 0.0) {
+      g[top] = g[top] / tri[top * ld + top];
     } else {
       s.reason = KSP_DIVERGED_BREAKDOWN;
       ok = false;
     }
     for (int k = top - 1; ok && k >= 0; --k) {
       double t = g[k];
-      for (int j = k + 1; j <= top; ++j) t = t - p.H[j * ld + k] * g[j];
-      if (p.H[k * ld + k] == 0.0) {
+      for (int j = k + 1; j <= top; ++j) t = t - tri[j * ld + k] * g[j];
+      if (tri[k * ld + k] == 0.0) {
         s.reason = KSP_DIVERGED_BREAKDOWN;
         ok = false;
         break;
       }
-      g[k] = t / p.H[k * ld + k];
+      g[k] = t / tri[k * ld + k];
     }
-    s.upd = ok ? 1 : 0;
+    if (ok) {
+      if (staged)
+        for (int k = 0; k <= top; ++k) p.grs[k] = g[k];
+      s.upd = 1;
+    }
   }
   if (!s.reason && s.its >= p.maxit) s.reason = KSP_DIVERGED_ITS;
   s.phase = s.reason ? PH_DONE : PH_RESTART;
@@ -756,7 +776,9 @@ void enqueue_round(int method, const Prob& p, int steps, hipStream_t s) {
   if (method == METHOD_GMRES) {
     const size_t dots_lds = sizeof(double) * (p.m + 1) * kWaves;
     const size_t rot_lds = sizeof(double) * 3 * (p.m + 1);
-    hipLaunchKernelGGL(k_g_solve, one, dim3(64), 0, s, p);
+    const size_t solve_lds = sizeof(double) * (size_t)(p.m + 1) * (p.m + 1) <= kSolveLds
+                                 ? sizeof(double) * (size_t)(p.m + 1) * (p.m + 1) : 0;
+    hipLaunchKernelGGL(k_g_solve, one, blk, solve_lds, s, p);
     hipLaunchKernelGGL(k_g_update, grid, blk, 0, s, p);
     hipLaunchKernelGGL(k_resid, sgrid, blk, 0, s, p, p.V, static_cast<double*>(nullptr));
     hipLaunchKernelGGL(k_g_start, one, rblk, 0, s, p);
