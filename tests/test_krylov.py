@@ -255,7 +255,7 @@ def assert_parity(method, A, b, **kw):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('method', METHODS)
-@pytest.mark.parametrize('nx,restart', [(12, 30), (12, 5), (40, 30), (64, 50)])
+@pytest.mark.parametrize('nx,restart', [(12, 30), (12, 5), (40, 30), (64, 50), (40, 100)])
 def test_hip_matches_oracle(method, nx, restart):
     if method == 'bcgs' and restart != 30:
         pytest.skip('restart is a GMRES parameter')
