@@ -261,6 +261,19 @@ class TestMatrices:
         op2.solve(vm, x, b)
         assert_allclose(x.data, mesh['f_vec'].data, 1e-12)
 
+    @pytest.mark.parametrize('method', ['gmres', 'bicgstab'])
+    def test_vector_solve_krylov(self, mesh, method):
+        """The vector mass system (2 x 2 blocks expanded into the scalar
+        CSR) through GMRES and BiCGStab."""
+        vm = vec_mass_mat(mesh)
+        b = op2.Dat(mesh['vnodes'], np.zeros((4, 2)), np.float64, "b")
+        op2.par_loop(op2.Kernel(RHS_VEC_TRI, "rhs_vec_tri"), mesh['elements'],
+                     b(mesh['elem_vnode'], op2.INC), mesh['coords'](mesh['elem_vnode'], op2.READ),
+                     mesh['f_vec'](mesh['elem_vnode'], op2.READ))
+        x = op2.Dat(mesh['vnodes'], np.zeros((4, 2)), np.float64, "x")
+        op2.Solver(linear_solver=method, relative_tolerance=1e-14).solve(vm, x, b)
+        assert_allclose(x.data, mesh['f_vec'].data, 1e-11)
+
     def test_zero_rhs(self, mesh):
         """A WRITE loop zeroes the right-hand side (reference
         test_matrices.py:693-697)."""
