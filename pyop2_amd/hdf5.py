@@ -13,10 +13,12 @@ Supported: superblock versions 0-3; object headers version 1 and 2 with
 continuation blocks; groups stored as symbol tables (B-tree version 1 and
 local heap) or as compact link messages; dataspaces (scalar and simple);
 fixed-point, floating-point and fixed-length string datatypes, either byte
-order; contiguous, compact and chunked (B-tree version 1) layouts, with the
-deflate, shuffle and Fletcher-32 filters; attributes with those types.
-Anything else (dense link or attribute storage, variable-length or compound
-types, other filters) raises NotImplementedError when it is reached.
+order, and variable-length strings and numeric sequences (global heap; as
+h5py stores Python strings); contiguous, compact and chunked (B-tree
+version 1) layouts, with the deflate, shuffle and Fletcher-32 filters;
+attributes with those types.  Anything else (dense link or attribute
+storage, compound or nested variable-length types, chunked variable-length
+data, other filters) raises NotImplementedError when it is reached.
 """
 
 import mmap
@@ -48,6 +50,28 @@ class _Reader(object):
         self.sl = length_size
         self.base = base            # file addresses are relative to the superblock's base
         self.undef = (1 << (8 * offset_size)) - 1
+        self._heaps = {}
+
+    def heap_object(self, addr, index):
+        """Bytes of object ``index`` of the global heap collection at
+        ``addr`` (variable-length data)."""
+        objs = self._heaps.get(addr)
+        if objs is None:
+            if self.bytes(addr, 4) != b'GCOL':
+                raise ValueError("HDF5: bad global heap collection")
+            end = addr + self.length(addr + 8)
+            p = addr + 8 + self.sl
+            objs = {}
+            while p + 8 + self.sl <= end:
+                idx = self.u(p, 2)
+                if idx == 0:                 # free space: the rest of the collection
+                    break
+                size = self.length(p + 8)
+                objs[idx] = (p + 8 + self.sl, size)
+                p += 8 + self.sl + _pad8(size)
+            self._heaps[addr] = objs
+        pos, size = objs[index]
+        return self.bytes(pos, size)
 
     def u(self, pos, size):
         return int.from_bytes(self.buf[pos:pos + size], 'little')
@@ -83,6 +107,7 @@ class _Datatype(object):
         bits = r.u(pos + 1, 3)
         self.size = r.u(pos + 4, 4)
         self.string = False
+        self.vlen = False
         if self.cls == 0:                      # fixed-point
             order = '>' if bits & 1 else '<'
             kind = 'i' if bits & 0x8 else 'u'
@@ -97,9 +122,16 @@ class _Datatype(object):
         elif self.cls == 3:                    # fixed-length string
             self.dtype = np.dtype('S%d' % self.size)
             self.string = True
+        elif self.cls == 9:                    # variable-length sequence or string
+            self.vlen_string = (bits & 0x0F) == 1
+            self.base = None if self.vlen_string else _Datatype(r, pos + 8)
+            if self.base is not None and (self.base.vlen or self.base.string):
+                raise NotImplementedError("HDF5: nested variable-length datatype")
+            self.dtype = np.dtype(object)
+            self.vlen = True
         else:
             names = {2: 'time', 4: 'bitfield', 5: 'opaque', 6: 'compound', 7: 'reference',
-                     8: 'enumerated', 9: 'variable-length', 10: 'array'}
+                     8: 'enumerated', 10: 'array'}
             raise NotImplementedError("HDF5: %s datatype" % names.get(self.cls, self.cls))
 
 
@@ -120,9 +152,27 @@ def _dataspace(r, pos):
     return tuple(r.length(p + k * r.sl) for k in range(ndims))
 
 
-def _decode(raw, dt, shape):
-    """numpy array (native byte order) from raw bytes of the given type."""
+def _decode(r, raw, dt, shape):
+    """numpy array (native byte order) from raw bytes of the given type;
+    variable-length elements (a 4-byte length and a global heap reference
+    each) become an object array of str or of 1-D arrays."""
     count = int(np.prod(shape)) if shape else 1
+    if dt.vlen:
+        out = np.empty(count, dtype=object)
+        step = 4 + r.so + 4
+        for k in range(count):
+            q = k * step
+            n = int.from_bytes(raw[q:q + 4], 'little')
+            addr = int.from_bytes(raw[q + 4:q + 4 + r.so], 'little')
+            idx = int.from_bytes(raw[q + 4 + r.so:q + step], 'little')
+            data = b'' if n == 0 or addr in (0, r.undef) else \
+                r.heap_object(addr + r.base, idx)
+            if dt.vlen_string:
+                out[k] = data[:n].rstrip(b'\0').decode('utf-8', 'replace')
+            else:
+                out[k] = np.frombuffer(data, dtype=dt.base.dtype, count=n).astype(
+                    dt.base.dtype.newbyteorder('='))
+        return out.reshape(shape)
     a = np.frombuffer(raw, dtype=dt.dtype, count=count)
     if dt.string:
         return a.reshape(shape)
@@ -318,7 +368,9 @@ class Attributes(object):
         if shape is None:
             return None
         count = int(np.prod(shape)) if shape else 1
-        a = _decode(r.bytes(p, count * dt.size), dt, shape)
+        a = _decode(r, r.bytes(p, count * dt.size), dt, shape)
+        if dt.vlen:
+            return a[()] if a.shape == () else a
         if dt.string:
             a = np.char.decode(np.char.rstrip(a.astype('S'), b'\0 '), 'utf-8')
             return str(a[()]) if a.shape == () else a
@@ -411,7 +463,7 @@ class Dataset(object):
                 raise NotImplementedError("HDF5: layout version %d class %d" % (version, cls))
         else:
             raise NotImplementedError("HDF5: layout version %d" % version)
-        return _decode(raw, self._dt, self.shape)
+        return _decode(r, raw, self._dt, self.shape)
 
     def _contiguous(self, addr, nbytes):
         if addr == self._r.undef:
@@ -447,13 +499,15 @@ class Dataset(object):
     def _chunked(self, btree, dims):
         """Read a chunked dataset through its B-tree (version 1, type 1)."""
         r = self._r
+        if self._dt.vlen:
+            raise NotImplementedError("HDF5: chunked variable-length dataset")
         cdims = dims[:-1]
         rank = len(cdims)
         esize = self._dt.size
         out = np.zeros(self.shape, dtype=self._dt.dtype)
         filters = self._filters()
         if btree == r.undef:
-            return _decode(out.tobytes(), self._dt, self.shape)
+            return _decode(r, out.tobytes(), self._dt, self.shape)
         ksize = 8 + 8 * (rank + 1)
 
         def node(addr):

@@ -4,8 +4,10 @@ Writes what the HDF5 library's defaults write for a flat file of
 datasets: superblock version 0, the root group as a symbol table (a
 version-1 B-tree over symbol-table nodes of at most 8 entries, names in a
 local heap), version-1 object headers, contiguous little-endian datasets of
-fixed-point or IEEE floating-point type, and fixed-length string or numeric
-attributes (HDF Group, "HDF5 File Format Specification", version 2).
+fixed-point or IEEE floating-point type, and fixed-length string, numeric
+or variable-length string attributes (``VlenStr``, as h5py stores Python
+strings; the strings go to one global heap collection) (HDF Group, "HDF5
+File Format Specification", version 2).
 The reader it feeds, pyop2_amd.hdf5, is checked separately against files
 written by the HDF5 library.
 """
@@ -21,6 +23,27 @@ INTERNAL_K = 16   # group internal node K: 2K = 32 children per B-tree node
 
 def _pad8(b):
     return b + b'\0' * (-len(b) % 8)
+
+
+class VlenStr(str):
+    """An attribute value stored as a variable-length string."""
+
+
+_VLEN_STR_TYPE = struct.pack('<B3sI', 0x19, bytes([0x01, 0, 0]), 16) + \
+    struct.pack('<B3sI', 0x10, bytes([0, 0, 0]), 1) + struct.pack('<HH', 0, 8)
+
+
+def _global_heap(strings):
+    """A global heap collection holding ``strings`` as objects 1..n, at
+    least 4096 bytes as the HDF5 library makes them."""
+    objs = b''
+    for k, t in enumerate(strings):
+        data = t.encode('utf-8')
+        objs += struct.pack('<HH4xQ', k + 1, 1, len(data)) + _pad8(data)
+    size = max(4096, 16 + len(objs) + 16)
+    free = size - 16 - len(objs)
+    return b'GCOL' + struct.pack('<B3xQ', 1, size) + objs + \
+        struct.pack('<HH4xQ', 0, 0, free) + b'\0' * (free - 16)
 
 
 def _datatype(dtype):
@@ -52,7 +75,15 @@ def _message(mtype, body):
     return struct.pack('<HHB3x', mtype, len(body), 0) + body
 
 
-def _attribute(name, value):
+def _attribute(name, value, heap=None):
+    if isinstance(value, VlenStr):
+        addr, index = heap[str(value)]
+        nm = name.encode('utf-8') + b'\0'
+        ds = _dataspace(())
+        data = struct.pack('<IQI', len(value.encode('utf-8')), addr, index)
+        body = struct.pack('<BBHHH', 1, 0, len(nm), len(_VLEN_STR_TYPE), len(ds)) + \
+            _pad8(nm) + _pad8(_VLEN_STR_TYPE) + _pad8(ds) + data
+        return _message(0x000C, body)
     if isinstance(value, str):
         raw = value.encode('utf-8') + b'\0'
         a = np.array(raw, dtype='S%d' % len(raw))
@@ -89,6 +120,13 @@ def write(path, datasets):
     for n in names:
         name_off[n] = len(heap)
         heap.extend(_pad8(n.encode('utf-8') + b'\0'))
+    # variable-length strings of the attributes: one global heap collection
+    vlen = sorted(set(str(v) for n in names for v in datasets[n][1].values()
+                      if isinstance(v, VlenStr)))
+    heap_refs = {}
+    if vlen:
+        gcol = place(_global_heap(vlen))
+        heap_refs = {t: (gcol, k + 1) for k, t in enumerate(vlen)}
     # datasets: raw data, then their object headers
     headers = {}
     for n in names:
@@ -99,7 +137,7 @@ def write(path, datasets):
         msgs = [_message(0x0001, _dataspace(a.shape)),
                 _message(0x0003, _datatype(a.dtype)),
                 _message(0x0008, struct.pack('<BBQQ', 3, 1, data_pos, a.nbytes))]
-        msgs += [_attribute(k, v) for k, v in sorted(attrs.items())]
+        msgs += [_attribute(k, v, heap_refs) for k, v in sorted(attrs.items())]
         headers[n] = place(_object_header(msgs))
     # symbol-table nodes of at most 2 * LEAF_K entries
     groups = [names[k:k + 2 * LEAF_K] for k in range(0, len(names), 2 * LEAF_K)]

@@ -71,6 +71,21 @@ def _read_all(g):
 
 
 @needs_samples
+def test_reader_vlen_string_attributes():
+    """Variable-length string attributes (their values as PyTables'
+    test_attributes.py states them)."""
+    with hdf5.File(os.path.join(SAMPLES, 'vlstr_attr.h5')) as f:
+        assert f.attrs['vlen_str_scalar'] == 'vlen_str_scalar'
+        a = f.attrs['vlen_str_array']
+        assert list(a) == ['vlen_str_array_%d' % k for k in range(len(a))]
+        m = f.attrs['vlen_str_matrix']
+        assert m.ndim == 2
+        for i in range(m.shape[0]):
+            for j in range(m.shape[1]):
+                assert m[i, j] == 'vlen_str_matrix_%d%d' % (i, j)
+
+
+@needs_samples
 def test_reader_unsupported_is_explicit():
     """A compound (table) dataset is refused with NotImplementedError."""
     with hdf5.File(os.path.join(SAMPLES, 'non-chunked-table.h5')) as f:
@@ -185,14 +200,16 @@ def test_airfoil_fromhdf5_hip(hip):
 class TestHDF5Backends:
     """The reference's test/unit/test_hdf5.py on every backend, its file
     made with the test writer instead of h5py (same datasets and
-    attributes)."""
+    attributes; the 'type' strings variable-length, as h5py stores them)."""
 
     @pytest.fixture(scope='class')
     def h5file(self, tmp_path_factory):
         p = str(tmp_path_factory.mktemp('h5') / 'tmp_hdf5.h5')
         h5write.write(p, {
-            'dat': (np.arange(10, dtype=np.float64).reshape(5, 2), {'type': 'double'}),
-            'soadat': (np.arange(10, dtype=np.float64).reshape(5, 2), {'type': 'double:soa'}),
+            'dat': (np.arange(10, dtype=np.float64).reshape(5, 2),
+                    {'type': h5write.VlenStr('double')}),
+            'soadat': (np.arange(10, dtype=np.float64).reshape(5, 2),
+                       {'type': h5write.VlenStr('double:soa')}),
             'set': (np.array((5,)), {'dim': 2}),
             'myconstant': (np.arange(3), {}),
             'map': (np.array((1, 2, 2, 3)).reshape(2, 2), {})})
