@@ -19,6 +19,9 @@
 // scales what it gathers by 1/|w| (the product PETSc's VecNormalize stores),
 // and the Gram-Schmidt dot kernel stores the scaled column.
 #include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
 
 #include "common.h"
 #include "reduce.h"
@@ -802,7 +805,7 @@ void enqueue_round(int method, const Prob& p, int steps, hipStream_t s) {
   }
 }
 
-// the capture stream and the instantiated round graphs of one solve
+// the capture stream and the instantiated round graphs of one problem
 struct Rounds {
   hipStream_t cap = nullptr;
   hipGraphExec_t exec[kChunk + 1] = {};
@@ -812,6 +815,56 @@ struct Rounds {
     if (cap) hipStreamDestroy(cap);
   }
 };
+
+// Round graphs kept across solves.  A graph's kernels take the Prob by
+// value, so a later solve whose Prob (matrix, vectors, work space, sizes)
+// is byte-for-byte the same on the same device replays the same graphs;
+// repeated solves of one system (time stepping) skip the capture.  The
+// least recently used of kMaxCached entries is dropped.  Not thread-safe,
+// like the rest of the library.
+constexpr size_t kMaxCached = 8;
+
+struct CacheEntry {
+  int method, device;
+  Prob p;
+  std::unique_ptr<Rounds> r;
+  unsigned long long used;
+};
+
+// never destroyed: graphs and streams must not be released during process
+// teardown, after the HIP runtime may have gone
+std::vector<CacheEntry>& round_cache() {
+  static std::vector<CacheEntry>* c = new std::vector<CacheEntry>();
+  return *c;
+}
+
+Rounds* cached_rounds(int method, int device, const Prob& p) {
+  static unsigned long long tick = 0;
+  auto& c = round_cache();
+  for (CacheEntry& e : c)
+    if (e.method == method && e.device == device && std::memcmp(&e.p, &p, sizeof(Prob)) == 0) {
+      e.used = ++tick;
+      return e.r.get();
+    }
+  if (c.size() >= kMaxCached) {
+    size_t old = 0;
+    for (size_t k = 1; k < c.size(); ++k)
+      if (c[k].used < c[old].used) old = k;
+    c.erase(c.begin() + old);
+  }
+  c.push_back(CacheEntry{method, device, p, std::unique_ptr<Rounds>(new Rounds()), ++tick});
+  return c.back().r.get();
+}
+
+// a cache entry whose capture failed part-way is not kept
+void drop_rounds(const Rounds* r) {
+  auto& c = round_cache();
+  for (size_t k = 0; k < c.size(); ++k)
+    if (c[k].r.get() == r) {
+      c.erase(c.begin() + k);
+      return;
+    }
+}
 
 }  // namespace
 
@@ -883,8 +936,17 @@ int op2hip_krylov_solve(int method, int n, const int* rowptr, const int* colidx,
   // Rounds of iterations replayed as hipGraphs (one per round length); the
   // state is read after each round.  Kernels outside their phase return at
   // once, so a round may run past a cycle's end or the solve's.
-  Rounds rounds;
-  OP2HIP_CHECK(hipStreamCreateWithFlags(&rounds.cap, hipStreamNonBlocking));
+  int device = 0;
+  OP2HIP_CHECK(hipGetDevice(&device));
+  Rounds& rounds = *cached_rounds(method, device, p);
+  if (!rounds.cap) {
+    const hipError_t e = hipStreamCreateWithFlags(&rounds.cap, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      rounds.cap = nullptr;
+      drop_rounds(&rounds);
+      OP2HIP_CHECK(e);
+    }
+  }
   const long long max_rounds = 4LL * (maxit > 0 ? maxit : 1) + 16;
   int st = OP2HIP_OK;
   for (long long round = 0;; ++round) {
@@ -896,13 +958,20 @@ int op2hip_krylov_solve(int method, int n, const int* rowptr, const int* colidx,
     hipGraphExec_t& g = rounds.exec[steps];
     if (!g) {
       hipGraph_t graph = nullptr;
-      OP2HIP_CHECK(hipStreamBeginCapture(rounds.cap, hipStreamCaptureModeThreadLocal));
-      enqueue_round(method, p, steps, rounds.cap);
-      hipError_t e = hipStreamEndCapture(rounds.cap, &graph);
-      OP2HIP_CHECK(e);
-      e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
-      OP2HIP_CHECK(e);
+      hipError_t e = hipStreamBeginCapture(rounds.cap, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        enqueue_round(method, p, steps, rounds.cap);
+        e = hipStreamEndCapture(rounds.cap, &graph);
+      }
+      if (e == hipSuccess) {
+        e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+      }
+      if (e != hipSuccess) {
+        g = nullptr;
+        drop_rounds(&rounds);
+        OP2HIP_CHECK(e);
+      }
     }
     OP2HIP_CHECK(hipGraphLaunch(g, s));
     st = read_state(p, &h, s);

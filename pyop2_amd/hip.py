@@ -443,14 +443,21 @@ class Solver(base.Solver):
             raise ValueError("gmres_restart must be at least 1")
         bdev = b._to_device()
         xdev = x._to_device()
-        nwork = nat.lib.op2hip_krylov_work(method, n, restart)
-        if nwork < 0:
-            raise ValueError("bad Krylov solve arguments")
-        work = torch.empty(max(int(nwork), 1), dtype=torch.float64, device=device())
         maxit = int(p['maximum_iterations'])
-        hist = None
-        if p['monitor_convergence']:
-            hist = torch.zeros(maxit + 1, dtype=torch.float64, device=device())
+        # the work space (and history) is kept between solves of one size, so
+        # repeated solves present the same arguments and replay the
+        # library's cached round graphs
+        key = (method, n, restart, maxit if p['monitor_convergence'] else None)
+        if getattr(self, '_kry', None) is None or self._kry[0] != key:
+            nwork = nat.lib.op2hip_krylov_work(method, n, restart)
+            if nwork < 0:
+                raise ValueError("bad Krylov solve arguments")
+            work = torch.empty(max(int(nwork), 1), dtype=torch.float64, device=device())
+            hist = None
+            if p['monitor_convergence']:
+                hist = torch.zeros(maxit + 1, dtype=torch.float64, device=device())
+            self._kry = (key, work, hist)
+        _, work, hist = self._kry
         its, rnorm, reason = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
         nat.check(nat.lib.op2hip_krylov_solve(
             method, n, ctypes.c_void_p(_ptr(sp._dev_rowptr)), ctypes.c_void_p(_ptr(sp._dev_colidx)),
