@@ -242,7 +242,9 @@ int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, 
  * doubles (-1 for bad arguments); x is overwritten (zero initial guess).
  * reshist (device, hist_len doubles, may be NULL with hist_len 0) receives
  * the monitor's residual norm per iteration.  its, rnorm and reason
- * (KSPConvergedReason) are host outputs.  Waits for the stream. */
+ * (KSPConvergedReason) are host outputs.  Waits for the stream.  The
+ * iterations run as captured hipGraphs, kept for the last 8 distinct
+ * argument lists and replayed by calls with the same arguments. */
 long long op2hip_krylov_work(int method, int n, int restart);
 int op2hip_krylov_solve(int method, int n, const int* rowptr, const int* colidx,
                         const double* vals, const double* b, double* x, double* work,
