@@ -3,7 +3,8 @@ as the reference's demos ask (demo/laplace_ffc.py:151, weak_bcs_ffc.py:182)
 and as PETSc 3.3's KSPGMRES / KSPBCGS with left PCJACOBI run them
 (pyop2/petsc_base.py:172-220; cusp names, pyop2/cuda.py:504-508).
 
-CPU: the oracle (oracle_gmres, oracle_bcgs) is pinned by
+CPU: the oracle (oracle_gmres, oracle_bcgs, and oracle_pcg's residual
+history) is pinned by
   * scipy's independent GMRES and BiCGStab on the explicitly preconditioned
     system D^-1 A x = D^-1 b: the same iteration counts and residual
     histories (GMRES estimates, BiCGStab true residuals);
@@ -415,3 +416,28 @@ def test_repeated_solves_replay_cached_graphs(hip, method):
     Am = A.values
     for bv, uv in ((b1, u1), (b2, u2)):
         assert np.abs(Am @ uv - bv).max() <= 1e-8 * np.abs(bv).max()
+
+
+def test_oracle_pcg_matches_scipy():
+    """The CG oracle (PETSc KSPCG with Jacobi, preconditioned norm) against
+    scipy's preconditioned CG on an SPD system: the same iteration count and
+    the same preconditioned residual norms |D^-1 (b - A x_k)| per
+    iteration."""
+    nx = 14
+    T = sps.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(nx, nx))
+    L0 = sps.kron(sps.identity(nx), T) + sps.kron(T, sps.identity(nx))
+    n = nx * nx
+    D = sps.diags(np.random.default_rng(9).uniform(0.5, 2.0, n))
+    A = (D @ L0 @ D).tocsr()                           # SPD, Jacobi not trivial
+    A.sort_indices()
+    b = np.random.default_rng(10).random(n)
+    x, its, rn, reason, hist = ora('pcg', A, b, rtol=1e-10)
+    assert reason == 2
+    dinv = 1.0 / A.diagonal()
+    xs = []
+    sl.cg(A, b, rtol=1e-14, atol=0.0, maxiter=its, M=sps.diags(dinv),
+          callback=lambda xk: xs.append(xk.copy()))
+    assert len(xs) == its
+    z = np.array([np.linalg.norm(dinv * (b - A @ xk)) for xk in xs])
+    assert close_history(hist[1:], z, 1e-8 * hist[0]) < 1e-6
+    assert np.abs(x - np.linalg.solve(A.toarray(), b)).max() <= 1e-8 * np.abs(x).max()
