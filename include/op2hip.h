@@ -210,6 +210,7 @@ typedef struct op2hip_cg_system {
   double rtol, atol, dtol;
   int maxit;
   int state; /* 0 empty, 1 new, 2 running, 3 done */
+  int no_precond; /* 0: Jacobi (PCJACOBI), 1: none (PCNONE) */
   double beta, betaold, dpi, dpiold, dp, rnorm0, ttol, alpha;
   int its, reason;
 } op2hip_cg_system;
@@ -237,8 +238,8 @@ int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, 
  * linear_solver 'gmres' or 'bcgs'/'bicgstab', pyop2/petsc_base.py:172-220;
  * the cusp backend's methods, pyop2/cuda.py:504-508): PETSc 3.3 KSPGMRES
  * (restarted every `restart` steps, classical Gram-Schmidt) or KSPBCGS,
- * with PCJACOBI on the left and the preconditioned residual norm.  method 1
- * is GMRES, 2 BiCGStab.  op2hip_krylov_work gives the device workspace in
+ * with PCJACOBI (jacobi 1) or PCNONE (jacobi 0) on the left and the
+ * preconditioned residual norm.  method 1 is GMRES, 2 BiCGStab.  op2hip_krylov_work gives the device workspace in
  * doubles (-1 for bad arguments); x is overwritten (zero initial guess).
  * reshist (device, hist_len doubles, may be NULL with hist_len 0) receives
  * the monitor's residual norm per iteration.  its, rnorm and reason
@@ -248,9 +249,9 @@ int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, 
 long long op2hip_krylov_work(int method, int n, int restart);
 int op2hip_krylov_solve(int method, int n, const int* rowptr, const int* colidx,
                         const double* vals, const double* b, double* x, double* work,
-                        long long work_len, int restart, double rtol, double atol, double dtol,
-                        int maxit, double* reshist, int hist_len, int* its, double* rnorm,
-                        int* reason, void* stream);
+                        long long work_len, int restart, int jacobi, double rtol, double atol,
+                        double dtol, int maxit, double* reshist, int hist_len, int* its,
+                        double* rnorm, int* reason, void* stream);
 
 #ifdef __cplusplus
 }

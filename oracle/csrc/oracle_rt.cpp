@@ -12,7 +12,8 @@
 //   oracle_pcg            PETSc 3.3 KSPSolve_CG + PCJACOBI + KSPConvergedDefault
 //                         (parameters from pyop2/base.py:1591-1601)
 //   oracle_gmres          PETSc 3.3 KSPGMRES (classical Gram-Schmidt, restart)
-//   oracle_bcgs           PETSc 3.3 KSPBCGS, both with left PCJACOBI
+//   oracle_bcgs           PETSc 3.3 KSPBCGS, all three with left PCJACOBI
+//                         (jacobi 1) or PCNONE (jacobi 0), the last argument
 //   oracle_plan_colour    the block colouring of pyop2/plan.pyx:214-376
 //                         (greedy 32-colour bit-mask rounds), used by the
 //                         OpenMP baseline
@@ -108,13 +109,13 @@ static double dot(int n, const double* a, const double* b) {
 // and KSPGetResidualNorm; reshist (max_it+1, may be NULL) the monitor values.
 int oracle_pcg(int n, const int* rp, const int* ci, const double* vals, const double* b, double* x,
                double rtol, double atol, double dtol, int max_it, int* its, double* rnorm,
-               double* reshist) {
+               double* reshist, int jacobi) {
   std::vector<double> inv(n), r(n), z(n), p(n), w(n);
   for (int i = 0; i < n; ++i) {
     double d = 0.0;
     for (int k = rp[i]; k < rp[i + 1]; ++k)
       if (ci[k] == i) d = vals[k];
-    inv[i] = d != 0.0 ? 1.0 / d : 1.0;
+    inv[i] = (jacobi && d != 0.0) ? 1.0 / d : 1.0;
   }
   for (int i = 0; i < n; ++i) {
     x[i] = 0.0;
@@ -189,12 +190,14 @@ static int ksp_test(int k, double rn, double rtol, double atol, double dtol, dou
   return 0;
 }
 
-static void jacobi_inverse(int n, const int* rp, const int* ci, const double* vals, double* inv) {
+// PCJACOBI's inverse diagonal (1 where it is zero), or 1 everywhere (PCNONE)
+static void jacobi_inverse(int n, const int* rp, const int* ci, const double* vals, double* inv,
+                           int jacobi) {
   for (int i = 0; i < n; ++i) {
     double d = 0.0;
     for (int k = rp[i]; k < rp[i + 1]; ++k)
       if (ci[k] == i) d = vals[k];
-    inv[i] = d != 0.0 ? 1.0 / d : 1.0;
+    inv[i] = (jacobi && d != 0.0) ? 1.0 / d : 1.0;
   }
 }
 
@@ -204,7 +207,7 @@ static void jacobi_inverse(int n, const int* rp, const int* ci, const double* va
 // estimate of the cycle before.
 int oracle_gmres(int n, const int* rp, const int* ci, const double* vals, const double* b,
                  double* x, int restart, double rtol, double atol, double dtol, int max_it,
-                 int* its, double* rnorm, double* reshist) {
+                 int* its, double* rnorm, double* reshist, int jacobi) {
   const int m = restart;
   const double haptol = 1e-30;
   std::vector<double> inv(n), tmp(n), V((size_t)(m + 1) * n);
@@ -212,7 +215,7 @@ int oracle_gmres(int n, const int* rp, const int* ci, const double* vals, const 
       h(m + 1);
   auto HH = [&](int i, int j) -> double& { return H[(size_t)j * (m + 1) + i]; };
   auto col = [&](int j) { return V.data() + (size_t)j * n; };
-  jacobi_inverse(n, rp, ci, vals, inv.data());
+  jacobi_inverse(n, rp, ci, vals, inv.data(), jacobi);
   for (int i = 0; i < n; ++i) x[i] = 0.0;
   int reason = 0;
   double rnorm0 = 0.0, ttol = 0.0, res = 0.0;
@@ -326,9 +329,9 @@ int oracle_gmres(int n, const int* rp, const int* ci, const double* vals, const 
 // KSPSolve_BCGS (left PCJACOBI, preconditioned norm).
 int oracle_bcgs(int n, const int* rp, const int* ci, const double* vals, const double* b,
                 double* x, double rtol, double atol, double dtol, int max_it, int* its,
-                double* rnorm, double* reshist) {
+                double* rnorm, double* reshist, int jacobi) {
   std::vector<double> inv(n), R(n), RP(n), V(n, 0.0), T(n), S(n), P(n, 0.0), tmp(n);
-  jacobi_inverse(n, rp, ci, vals, inv.data());
+  jacobi_inverse(n, rp, ci, vals, inv.data(), jacobi);
   for (int i = 0; i < n; ++i) {
     x[i] = 0.0;
     R[i] = inv[i] * b[i];

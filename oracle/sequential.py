@@ -201,7 +201,7 @@ class Solver(base.Solver):
     def solve(self, A, x, b):
         p = self.parameters
         method = KRYLOV_NAMES.get(p['linear_solver'])
-        assert method is not None and p['preconditioner'] == 'jacobi'
+        assert method is not None and p['preconditioner'] in ('jacobi', 'none')
         sp = A.sparsity
         n = sp.nrows * A.dims[0]
         bb = np.ascontiguousarray(b.data_ro, dtype=np.float64).reshape(-1)
@@ -214,13 +214,14 @@ class Solver(base.Solver):
                float(p['divergence_tolerance'])]
         head = [n, sp._rowptr.ctypes.data, sp._colidx.ctypes.data, A._array.ctypes.data,
                 bb.ctypes.data, xx.ctypes.data]
-        tail = [maxit, ctypes.byref(its), ctypes.byref(rn), hist.ctypes.data]
+        tail = [maxit, ctypes.byref(its), ctypes.byref(rn), hist.ctypes.data,
+                int(p['preconditioner'] == 'jacobi')]
         f = getattr(rt(), 'oracle_' + method)
         f.restype = ctypes.c_int
         extra = [int(p.get('gmres_restart', 30))] if method == 'gmres' else []
         f.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] * len(extra) + \
             [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-                                     ctypes.c_void_p]
+                                     ctypes.c_void_p, ctypes.c_int]
         reason = f(*(head + extra + tol + tail))
         x.data[:] = xx.reshape(x.data.shape)
         self._its, self._rnorm, self._reason = its.value, rn.value, reason

@@ -118,8 +118,8 @@ _SIGS = {
     'op2hip_cgb_chunk_rows': (c_int, []),
     'op2hip_krylov_work': (c_ll, [c_int, c_int, c_int]),
     'op2hip_krylov_solve': (c_int, [c_int, c_int, c_void, c_void, c_void, c_void, c_void, c_void,
-                                    c_ll, c_int, c_double, c_double, c_double, c_int, c_void,
-                                    c_int, P(c_int), P(c_double), P(c_int), c_void]),
+                                    c_ll, c_int, c_int, c_double, c_double, c_double, c_int,
+                                    c_void, c_int, P(c_int), P(c_double), P(c_int), c_void]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

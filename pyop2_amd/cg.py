@@ -29,7 +29,7 @@ class CGSystem(ctypes.Structure):
                 ('inv_diag', _p), ('r', _p), ('z', _p), ('p', _p), ('w', _p),
                 ('row0', _i), ('n', _i), ('chunk0', _i), ('nchunks', _i),
                 ('rtol', _d), ('atol', _d), ('dtol', _d), ('maxit', _i), ('state', _i),
-                ('beta', _d), ('betaold', _d), ('dpi', _d), ('dpiold', _d), ('dp', _d),
+                ('no_precond', _i), ('beta', _d), ('betaold', _d), ('dpi', _d), ('dpiold', _d), ('dp', _d),
                 ('rnorm0', _d), ('ttol', _d), ('alpha', _d), ('its', _i), ('reason', _i)]
 
 
@@ -43,16 +43,19 @@ def chunk_rows():
     return CHUNK
 
 
-def make_system(rowptr, colidx, vals, b, x, work, row0, n, rtol, atol, dtol, maxit):
+def make_system(rowptr, colidx, vals, b, x, work, row0, n, rtol, atol, dtol, maxit,
+                jacobi=True):
     """A NEW system over rows [row0, row0+n).  ``work`` holds 5 device
     vectors (inv_diag, r, z, p, w) laid out like x (indexed by global row):
-    pass the base addresses of 5 arrays of the global length."""
+    pass the base addresses of 5 arrays of the global length.  ``jacobi``
+    False runs CG without a preconditioner (PETSc's PCNONE)."""
     s = CGSystem()
     s.rowptr, s.colidx, s.vals, s.b, s.x = rowptr, colidx, vals, b, x
     s.inv_diag, s.r, s.z, s.p, s.w = work
     s.row0, s.n = row0, n
     s.rtol, s.atol, s.dtol, s.maxit = rtol, atol, dtol, maxit
     s.state = SYS_NEW
+    s.no_precond = 0 if jacobi else 1
     return s
 
 

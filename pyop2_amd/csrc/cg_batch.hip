@@ -68,6 +68,7 @@ struct op2hip_cg_system {
   double rtol, atol, dtol;
   int maxit;
   int state;
+  int no_precond;  // 0: Jacobi, 1: none (D^-1 taken as I)
   // solver scalars
   double beta, betaold, dpi, dpiold, dp, rnorm0, ttol, alpha;
   int its, reason;
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunk
     double d = 0.0;
     for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
       if (colidx[k] == i) d = vals[k];
-    double inv = d != 0.0 ? 1.0 / d : 1.0;
+    double inv = (!s.no_precond && d != 0.0) ? 1.0 / d : 1.0;
     inv_diag[i] = inv;
     double ri = b[i];
     double zi = inv * ri;

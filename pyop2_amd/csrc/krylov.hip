@@ -82,6 +82,7 @@ struct Prob {
   double* part;  // partials by quantity: quantity k of workgroup b at k * ps + b
   double* hist;  // residual history (hist_len) or null
   int n, lanes, ps, m, maxit, hist_len;  // ps = max(nb, nbs)
+  int jacobi;  // 1: D^-1 is the inverse diagonal (PCJACOBI); 0: I (PCNONE)
   int nb;   // workgroups of the tile and element kernels (kTile rows each)
   int nbs;  // workgroups of the SpMV kernels (more, for gather latency)
   // GMRES
@@ -175,7 +176,8 @@ __device__ inline void spmv_rows(const Prob& p, G gather, F finish) {
 
 // --- shared --------------------------------------------------------------------
 
-// Jacobi inverse diagonal (1 where the diagonal is zero or absent) and x = 0
+// Jacobi inverse diagonal (1 where the diagonal is zero or absent; 1
+// everywhere without a preconditioner) and x = 0
 __global__ void __launch_bounds__(kBlock) k_init(Prob p) {
   const int* __restrict__ rowptr = p.rowptr;
   const int* __restrict__ colidx = p.colidx;
@@ -183,7 +185,7 @@ __global__ void __launch_bounds__(kBlock) k_init(Prob p) {
     double d = 0.0;
     for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
       if (colidx[k] == i) d = p.vals[k];
-    p.inv[i] = d != 0.0 ? 1.0 / d : 1.0;
+    p.inv[i] = (p.jacobi && d != 0.0) ? 1.0 / d : 1.0;
     p.x[i] = 0.0;
   }
 }
@@ -877,9 +879,9 @@ long long op2hip_krylov_work(int method, int n, int restart) {
 
 int op2hip_krylov_solve(int method, int n, const int* rowptr, const int* colidx,
                         const double* vals, const double* b, double* x, double* work,
-                        long long work_len, int restart, double rtol, double atol, double dtol,
-                        int maxit, double* reshist, int hist_len, int* its, double* rnorm,
-                        int* reason, void* stream) {
+                        long long work_len, int restart, int jacobi, double rtol, double atol,
+                        double dtol, int maxit, double* reshist, int hist_len, int* its,
+                        double* rnorm, int* reason, void* stream) {
   OP2HIP_REQUIRE(method == METHOD_GMRES || method == METHOD_BCGS, "unknown Krylov method %d",
                  method);
   OP2HIP_REQUIRE(n >= 0 && restart >= 0 && its && rnorm && reason, "bad Krylov arguments");
@@ -911,6 +913,7 @@ int op2hip_krylov_solve(int method, int n, const int* rowptr, const int* colidx,
   p.n = n;
   p.m = restart;
   p.maxit = maxit;
+  p.jacobi = jacobi ? 1 : 0;
   // lanes per SpMV row from the mean row length: about two entries a lane
   int nnz = 0;
   OP2HIP_CHECK(hipMemcpyAsync(&nnz, rowptr + n, sizeof(int), hipMemcpyDeviceToHost, s));

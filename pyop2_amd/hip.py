@@ -405,9 +405,9 @@ class Solver(base.Solver):
         if method is None:
             raise NotImplementedError("linear_solver %r: the hip backend implements %s"
                                       % (p['linear_solver'], sorted(_KRYLOV)))
-        if p['preconditioner'] not in ('jacobi',):
-            raise NotImplementedError("preconditioner %r: the hip backend implements 'jacobi'"
-                                      % p['preconditioner'])
+        if p['preconditioner'] not in ('jacobi', 'none'):
+            raise NotImplementedError("preconditioner %r: the hip backend implements 'jacobi' "
+                                      "and 'none'" % p['preconditioner'])
         if p['plot_convergence']:
             p['monitor_convergence'] = True
         if method:
@@ -423,7 +423,8 @@ class Solver(base.Solver):
         sysd = cg.make_system(_ptr(sp._dev_rowptr), _ptr(sp._dev_colidx), _ptr(A._dev_vals),
                               _ptr(bdev), _ptr(xdev), [_ptr(work[k]) for k in range(5)], 0, n,
                               float(p['relative_tolerance']), float(p['absolute_tolerance']),
-                              float(p['divergence_tolerance']), maxit)
+                              float(p['divergence_tolerance']), maxit,
+                              jacobi=p['preconditioner'] == 'jacobi')
         hist_len = maxit + 1 if p['monitor_convergence'] else 0
         batch = cg.CGBatch(1, n, hist_len)
         out = batch.solve([sysd], s, chunk=int(p.get('hip_cg_chunk', 16)))[0]
@@ -463,7 +464,7 @@ class Solver(base.Solver):
             method, n, ctypes.c_void_p(_ptr(sp._dev_rowptr)), ctypes.c_void_p(_ptr(sp._dev_colidx)),
             ctypes.c_void_p(_ptr(A._dev_vals)), ctypes.c_void_p(_ptr(bdev)),
             ctypes.c_void_p(_ptr(xdev)), ctypes.c_void_p(_ptr(work)), int(work.numel()), restart,
-            float(p['relative_tolerance']), float(p['absolute_tolerance']),
+            1 if p['preconditioner'] == 'jacobi' else 0, float(p['relative_tolerance']), float(p['absolute_tolerance']),
             float(p['divergence_tolerance']), maxit,
             ctypes.c_void_p(_ptr(hist) if hist is not None else None),
             0 if hist is None else int(hist.numel()), ctypes.byref(its), ctypes.byref(rnorm),

@@ -62,7 +62,7 @@ def convdiff(nx, bx=30.0, by=15.0, seed=0, shift=0.0):
     return A
 
 
-def ora(method, A, b, rtol=1e-10, atol=1e-50, dtol=1e4, maxit=1000, restart=30):
+def ora(method, A, b, rtol=1e-10, atol=1e-50, dtol=1e4, maxit=1000, restart=30, jacobi=1):
     """The oracle's solve: (x, its, rnorm, reason, history)."""
     rp = A.indptr.astype(np.int32)
     ci = A.indices.astype(np.int32)
@@ -76,10 +76,10 @@ def ora(method, A, b, rtol=1e-10, atol=1e-50, dtol=1e4, maxit=1000, restart=30):
     f.restype = ctypes.c_int
     extra = [restart] if method == 'gmres' else []
     f.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 5 + [ctypes.c_int] * len(extra) + \
-        [ctypes.c_double] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 3
+        [ctypes.c_double] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int]
     reason = f(n, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, bb.ctypes.data, x.ctypes.data,
                *extra, rtol, atol, dtol, maxit, ctypes.byref(its), ctypes.byref(rn),
-               hist.ctypes.data)
+               hist.ctypes.data, jacobi)
     return x, its.value, rn.value, reason, hist[:its.value + 1].copy()
 
 
@@ -208,7 +208,7 @@ def test_oracle_exits(method):
 # -- GPU: the C-ABI solve against the oracle --------------------------------------
 
 def hip_solve(method, A, b, rtol=1e-10, atol=1e-50, dtol=1e4, maxit=1000, restart=30,
-              hist=True):
+              hist=True, jacobi=1):
     import torch
     from pyop2_amd import _native as nat
     dev = torch.device('cuda', 0)
@@ -227,7 +227,7 @@ def hip_solve(method, A, b, rtol=1e-10, atol=1e-50, dtol=1e4, maxit=1000, restar
     s = torch.cuda.current_stream().cuda_stream
     nat.check(nat.lib.op2hip_krylov_solve(
         code, n, rp.data_ptr(), ci.data_ptr(), v.data_ptr(), bd.data_ptr(), x.data_ptr(),
-        work.data_ptr(), work.numel(), restart, rtol, atol, dtol, maxit,
+        work.data_ptr(), work.numel(), restart, jacobi, rtol, atol, dtol, maxit,
         h.data_ptr() if hist else None, h.numel() if hist else 0, ctypes.byref(its),
         ctypes.byref(rn), ctypes.byref(reason), s), 'krylov')
     its = its.value
@@ -482,3 +482,75 @@ def test_op2_assembled_spd_cg(hip):
     assert rg == ro and rg > 0 and abs(ig - io) <= 1
     assert close_history(hg, ho, 1e-8 * ho[0]) < 1e-5
     assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()
+
+
+# -- no preconditioner (PETSc's PCNONE, preconditioner='none') --------------------
+
+@pytest.mark.parametrize('method', ['gmres', 'bcgs', 'pcg'])
+def test_oracle_no_preconditioner_matches_scipy(method):
+    """Without a preconditioner the residual norms are |b - A x_k|: the
+    oracle's histories against scipy's unpreconditioned solvers."""
+    if method == 'pcg':
+        nx = 12
+        T = sps.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(nx, nx))
+        A = (sps.kron(sps.identity(nx), T) + sps.kron(T, sps.identity(nx))).tocsr()
+    else:
+        A = convdiff(10, seed=4)
+    A.sort_indices()
+    b = np.random.default_rng(12).random(A.shape[0])
+    x, its, rn, reason, hist = ora(method, A, b, rtol=1e-10, jacobi=0)
+    assert reason == 2
+    got = []
+    if method == 'gmres':
+        sl.gmres(A, b, rtol=1e-10, atol=0.0, restart=30, maxiter=500, callback=got.append,
+                 callback_type='pr_norm')
+        ref = np.array(got) * np.linalg.norm(b)
+    else:
+        solver = sl.cg if method == 'pcg' else sl.bicgstab
+        solver(A, b, rtol=1e-14, atol=0.0, maxiter=its, callback=lambda xk: got.append(xk.copy()))
+        ref = np.array([np.linalg.norm(b - A @ xk) for xk in got])
+    assert len(ref) == its
+    assert close_history(hist[1:], ref, 1e-8 * hist[0]) < 1e-5
+    assert np.abs(x - np.linalg.solve(A.toarray(), b)).max() <= 1e-8 * np.abs(x).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('method', METHODS)
+def test_hip_no_preconditioner(method):
+    A = convdiff(24, seed=24)
+    b = np.random.default_rng(24).random(A.shape[0])
+    assert_parity(method, A, b, rtol=1e-12, jacobi=0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('linear_solver', ['cg', 'gmres', 'bicgstab'])
+def test_op2_no_preconditioner(hip, linear_solver):
+    """Solver(preconditioner='none') on the hip backend against the oracle
+    on the reference's 2-triangle-free SPD case: a P1 mass matrix."""
+    from pyop2_amd.workloads import fem, meshes
+    mesh = meshes.TriMesh(12, seed=8)
+    fv = np.random.default_rng(8).random(mesh.nvert)
+
+    def run(W):
+        nodes, vnodes = W.Set(mesh.nvert, 1), W.Set(mesh.nvert, 2)
+        el = W.Set(mesh.nelem, 1)
+        en = W.Map(el, nodes, 3, mesh.elem_node)
+        ev = W.Map(el, vnodes, 3, mesh.elem_node)
+        A = W.Mat(W.Sparsity((en, en)), np.float64)
+        x = W.Dat(vnodes, mesh.coords.copy(), np.float64)
+        f = W.Dat(nodes, fv.copy(), np.float64)
+        b = W.Dat(nodes, np.zeros(mesh.nvert), np.float64)
+        u = W.Dat(nodes, np.zeros(mesh.nvert), np.float64)
+        W.par_loop(W.Kernel(fem.MASS_TRI, 'mass_tri'), el(3, 3),
+                   A((en[W.i[0]], en[W.i[1]]), W.INC), x(ev, W.READ))
+        W.par_loop(W.Kernel(fem.RHS_TRI, 'rhs_tri'), el(3), b(en[W.i[0]], W.INC),
+                   x(ev, W.READ), f(en, W.READ))
+        s = W.Solver(linear_solver=linear_solver, preconditioner='none',
+                     relative_tolerance=1e-12, maximum_iterations=5000)
+        s.solve(A, u, b)
+        return np.array(u.data_ro).ravel(), s.getIterationNumber(), s.getConvergedReason()
+
+    ug, ig, rg = run(hip)
+    uo, io, ro = run(seq)
+    assert rg == ro and rg > 0 and abs(ig - io) <= 1
+    assert np.abs(ug - fv).max() <= 1e-8 and np.abs(ug - uo).max() <= 1e-9
