@@ -1,17 +1,18 @@
-"""GMRES and BiCGStab: Solver(linear_solver='gmres' | 'bcgs' | 'bicgstab'),
-as the reference's demos ask (demo/laplace_ffc.py:151, weak_bcs_ffc.py:182)
-and as PETSc 3.3's KSPGMRES / KSPBCGS with left PCJACOBI run them
+"""Krylov solves: Solver(linear_solver='cg' | 'gmres' | 'bcgs' | 'bicgstab',
+preconditioner='jacobi' | 'none'), as the reference's demos ask
+(demo/laplace_ffc.py:151, weak_bcs_ffc.py:182) and as PETSc 3.3's KSPCG,
+KSPGMRES and KSPBCGS with left PCJACOBI or PCNONE run them
 (pyop2/petsc_base.py:172-220; cusp names, pyop2/cuda.py:504-508).
 
-CPU: the oracle (oracle_gmres, oracle_bcgs, and oracle_pcg's residual
-history) is pinned by
-  * scipy's independent GMRES and BiCGStab on the explicitly preconditioned
-    system D^-1 A x = D^-1 b: the same iteration counts and residual
-    histories (GMRES estimates, BiCGStab true residuals);
+CPU: the oracle (oracle_gmres, oracle_bcgs, oracle_pcg) is pinned by
+  * scipy's independent GMRES, BiCGStab and CG on the explicitly
+    preconditioned system D^-1 A x = D^-1 b (CG: scipy's PCG with M = D^-1),
+    and unpreconditioned: the same iteration counts and residual histories
+    (GMRES estimates, BiCGStab and CG true residuals);
   * brute force: the GMRES residual after k steps is the minimum of
-    |D^-1 (b - A x)| over x in the k-th Krylov space, computed by least squares
-    on a twice-orthogonalised basis, in the first cycle and, from the
-    restarted iterate, in the second;
+    |D^-1 (b - A x)| over x in the k-th Krylov space, computed by least
+    squares on a twice-orthogonalised basis, in the first cycle and, from
+    the restarted iterate, in the second;
   * closed cases: zero right-hand side (ATOL at 0 iterations), a diagonal
     matrix with power-of-two entries (BiCGStab's t = 0 exit gives x = b/d
     exactly at 1 iteration), max_it and dtol exits, the dense solution.
@@ -19,12 +20,14 @@ GPU: the C-ABI solve against the oracle on the same systems (reason equal,
 iterations equal or one apart, as the last residual may cross the tolerance
 a step later under another summation order; residual histories within 1e-5
 relative for GMRES, 1e-4 above 1e-6 of the first residual for BiCGStab;
-solutions within 1e-8 of the oracle's at rtol 1e-12, as both
-meet the same residual bound); op2-level solves of an assembled
-nonsymmetric P1 convection-diffusion operator with Dirichlet rows
-(zero_rows) against the oracle backend (same reason and solution, first
-iterations alike, iteration counts within a tenth for BiCGStab); at 1M rows the true residual
-equals the reported one."""
+solutions within 1e-8 of the oracle's at rtol 1e-12, as both meet the same
+residual bound), with and without the preconditioner; op2-level solves of
+an assembled nonsymmetric P1 convection-diffusion operator with Dirichlet
+rows (zero_rows) and of SPD operators (CG) against the oracle backend (same
+reason and solution, histories alike; iteration counts within a tenth for
+BiCGStab on the convection-dominated case); repeated solves replaying the
+library's cached graphs; at 1M rows the true residual equals the reported
+one."""
 
 import ctypes
 
