@@ -78,13 +78,20 @@ def parse():
                    help='targets owned per block of owner-computes loops (0: sized automatically)')
     p.add_argument('--assembly', type=int, default=128,
                    help='edge count n of the n^3 Kuhn-tet assembly measurement (0: skip)')
-    p.add_argument('--batch', type=int, default=64,
+    p.add_argument('--batch', type=int, default=256,
                    help='meshes in the batched assembly+CG measurement (0: skip)')
-    p.add_argument('--batch-emax', type=float, default=2e5)
-    p.add_argument('--serve', type=int, default=48,
+    p.add_argument('--batch-emin', type=float, default=2e3)
+    p.add_argument('--batch-emax', type=float, default=2e6)
+    p.add_argument('--batch-rounds', type=int, default=1,
+                   help='timed rounds of each batching path (after one untimed round)')
+    p.add_argument('--serve', type=int, default=64,
                    help='requests per arrival rate in the serving measurement (0: skip)')
-    p.add_argument('--serve-pool', type=int, default=4,
+    p.add_argument('--serve-pool', type=int, default=32,
                    help='2-D and 3-D meshes each in the serving pool')
+    p.add_argument('--serve-tri', type=float, nargs=2, default=(8e3, 5e5),
+                   help='element range of the pool\'s triangle meshes')
+    p.add_argument('--serve-tet', type=float, nargs=2, default=(5e4, 2e6),
+                   help='element range of the pool\'s tetrahedral meshes')
     p.add_argument('--solvers', type=float, default=2e6,
                    help='elements of the 2-D system the CG, GMRES(30) and BiCGStab '
                         'iterations are timed on (0: skip)')
@@ -287,18 +294,34 @@ def main():
         extra['assembly'] = assembly(op2, args.assembly)
     if args.batch:
         from pyop2_amd.workloads import suites
-        extra['batched'] = suites.batch_suite(op2, nmesh=args.batch, emax=args.batch_emax)
+        extra['batched'] = suites.batch_suite(op2, nmesh=args.batch, emin=args.batch_emin,
+                                              emax=args.batch_emax, rounds=args.batch_rounds,
+                                              progress=_progress('batch'))
     if args.serve:
         from pyop2_amd.workloads import suites
         pool = suites.make_pool(op2, n2d=args.serve_pool, n3d=args.serve_pool,
-                                tri_range=(8000, 100000), tet_range=(50000, 200000))
-        extra['serving'] = suites.serve_suite(op2, pool, nreq=args.serve)
+                                tri_range=tuple(args.serve_tri),
+                                tet_range=tuple(args.serve_tet))
+        _progress('serve')('pool of %d meshes resident' % len(pool))
+        extra['serving'] = suites.serve_suite(op2, pool, nreq=args.serve,
+                                              progress=_progress('serve'))
     if args.solvers:
         from pyop2_amd.workloads import cgbench
         extra['solvers'] = cgbench.run(op2, int(args.solvers), 200)
     if extra:
         out['extra'] = extra
     print(json.dumps(out))
+
+
+def _progress(leg):
+    """A progress callback writing to stderr (stdout carries only the JSON
+    line)."""
+    t0 = time.perf_counter()
+
+    def say(msg):
+        sys.stderr.write('[bench %s %.1fs] %s\n' % (leg, time.perf_counter() - t0, msg))
+        sys.stderr.flush()
+    return say
 
 
 def _tuned():

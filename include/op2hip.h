@@ -160,12 +160,19 @@ int op2hip_order_first_touch(int ntarget, int nsrc, const int* map, int dim,
  * build writes rowptr (device, nrows*rmult+1) and *nnz_out; fill writes
  * colidx (device, nnz) and nnz_rows (device, nrows*rmult, may be NULL) and
  * releases the handle.  rowmaps/colmaps: host arrays of device int32 map
- * pointers; nelems[m]: iteration-set size of pair m.  Bit-exact to the
- * reference's sorted-set construction. */
+ * pointers (at most 8 map pairs); nelems[m]: iteration-set size of pair m.
+ * work: a device buffer of op2hip_sparsity_workspace() bytes (about
+ * 4 * (4 * nrows + sum of nelems[m] * rdims[m])), kept untouched until the
+ * handle is filled or discarded; the build allocates no device memory.
+ * Build waits for the stream once (nnz), fill once (its check that every
+ * row recounted to the build's count).  Bit-exact to the reference's
+ * sorted-set construction. */
+long long op2hip_sparsity_workspace(int nrows, int nmaps, const long long* nelems,
+                                    const int* rdims);
 int op2hip_sparsity_build(int rmult, int cmult, int nrows, int nmaps, const int* const* rowmaps,
                           const int* rdims, const int* const* colmaps, const int* cdims,
-                          const long long* nelems, int* rowptr, void** handle,
-                          long long* nnz_out, void* stream);
+                          const long long* nelems, void* work, long long work_bytes,
+                          int* rowptr, void** handle, long long* nnz_out, void* stream);
 int op2hip_sparsity_fill(void* handle, const int* rowptr, int* colidx, int* nnz_rows,
                          void* stream);
 int op2hip_sparsity_discard(void* handle);

@@ -101,8 +101,10 @@ _SIGS = {
                            P(ctypes.c_double), P(c_int), c_void]),
     'op2hip_order_bisect': (c_int, [c_int, c_int, c_void, c_int, c_int, c_void]),
     'op2hip_order_first_touch': (c_int, [c_int, c_int, c_void, c_int, c_void, c_void]),
+    'op2hip_sparsity_workspace': (c_ll, [c_int, c_int, P(c_ll), P(c_int)]),
     'op2hip_sparsity_build': (c_int, [c_int, c_int, c_int, c_int, P(c_void), P(c_int), P(c_void),
-                                      P(c_int), P(c_ll), c_void, P(c_void), P(c_ll), c_void]),
+                                      P(c_int), P(c_ll), c_void, c_ll, c_void, P(c_void), P(c_ll),
+                                      c_void]),
     'op2hip_sparsity_fill': (c_int, [c_void, c_void, c_void, c_void, c_void]),
     'op2hip_sparsity_discard': (c_int, [c_void]),
     'op2hip_spmv': (c_int, [c_int, c_ll, c_void, c_void, c_void, c_void, c_void, c_void]),

@@ -442,8 +442,11 @@ class Dat(DataCarrier):
             self._id = uid
         self._name = name or "dat_%d" % self._id
 
-    @validate_in(('access', _modes, ModeValueError))
     def __call__(self, path, access):
+        return self._arg(canonical(path), canonical(access))
+
+    @validate_in(('access', _modes, ModeValueError))
+    def _arg(self, path, access):
         if isinstance(path, Map):
             if path._dataset != self._dataset and path != IdentityMap:
                 raise MapValueError("Dataset of Map does not match Dataset of Dat.")
@@ -681,8 +684,11 @@ class Global(DataCarrier):
         self._name = name or "global_%d" % Global._globalcount
         Global._globalcount += 1
 
-    @validate_in(('access', _modes, ModeValueError))
     def __call__(self, access):
+        return self._arg(canonical(access))
+
+    @validate_in(('access', _modes, ModeValueError))
+    def _arg(self, access):
         return _new_arg(self, data=self, access=access)
 
     def __str__(self):
@@ -778,7 +784,7 @@ class Map(object):
     def __getitem__(self, index):
         if isinstance(index, slice):
             raise NotImplementedError("Slicing maps is not allowed")
-        return self._index(index)
+        return self._index(canonical(index))
 
     @validate_type(('index', (int, IterationIndex), IndexTypeError))
     def _index(self, index):
@@ -837,6 +843,36 @@ class Map(object):
 
 
 IdentityMap = Map(Set(0), Set(0), 1, [], 'identity')
+
+_ACCESS = {a._mode: a for a in (READ, WRITE, RW, INC, MIN, MAX)}
+
+
+def canonical(obj):
+    """This module's descriptor for one made by another implementation of
+    the op2 API; other objects are returned unchanged.
+
+    A backend plugged into the reference's own front end receives the
+    reference's descriptors: its op2 module exports READ ... MAX,
+    IdentityMap and i from its base module (pyop2/op2.py:43, defined at
+    pyop2/base.py:56-90, 981-1011, 1127), and user code passes them to the
+    Dats, Maps and Mats this backend made.  They are matched by role, not
+    identity: an access descriptor by its mode name, an iteration index by
+    its index, and the identity map as a map named 'identity' of arity 1
+    from an empty Set.  Everything past this point compares descriptors by
+    identity against this module's singletons."""
+    if obj is None or isinstance(obj, (Access, IterationIndex, Map, Arg, int)):
+        return obj
+    kind = type(obj).__name__
+    if kind == 'Access':
+        return _ACCESS.get(getattr(obj, '_mode', None), obj)
+    if kind == 'IterationIndex' and hasattr(obj, 'index'):
+        idx = obj.index
+        return i if idx is None else IterationIndex(idx)
+    if kind == 'Map' and getattr(obj, 'name', None) == 'identity' and \
+            getattr(obj, 'dim', None) == 1 and \
+            getattr(getattr(obj, 'iterset', None), 'size', None) == 0:
+        return IdentityMap
+    return obj
 
 
 class Sparsity(Cached):
@@ -966,8 +1002,11 @@ class Mat(DataCarrier):
         self._name = name or "mat_%d" % Mat._globalcount
         Mat._globalcount += 1
 
-    @validate_in(('access', _modes, ModeValueError))
     def __call__(self, path, access):
+        return self._arg(path, canonical(access))
+
+    @validate_in(('access', _modes, ModeValueError))
+    def _arg(self, path, access):
         path = as_tuple(path, Arg, 2)
         path_maps = [arg.map for arg in path]
         path_idxs = [arg.idx for arg in path]

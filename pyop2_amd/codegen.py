@@ -34,7 +34,7 @@ the min/max (reference semantics: pyop2/base.py:216-248).
 
 import numpy as np
 
-from .base import IterationIndex, READ, WRITE, RW, INC, MIN, MAX
+from .base import IterationIndex, READ, WRITE, RW, INC, MIN, MAX, canonical
 
 
 # contribution codes an owner write-back loads together per target
@@ -115,8 +115,10 @@ _REDUCE_OPS = {'INC': ('a + b', '0'), 'MIN': ('(b < a ? b : a)', None),
 
 
 def _mode(access):
+    """Mode name of an access descriptor (this module's or, matched by
+    name, another op2 implementation's)."""
     return {id(READ): 'READ', id(WRITE): 'WRITE', id(RW): 'RW', id(INC): 'INC',
-            id(MIN): 'MIN', id(MAX): 'MAX'}[id(access)]
+            id(MIN): 'MIN', id(MAX): 'MAX'}[id(canonical(access))]
 
 
 class LoopSource(object):

@@ -840,11 +840,29 @@ std::vector<CacheEntry>& round_cache() {
   return *c;
 }
 
+// Field-by-field equality of two argument blocks.  Prob has padding (nine
+// ints before the GMRES pointers), whose bytes the language
+// leaves unspecified, so a byte comparison of whole structs is not a sound
+// cache key.
+bool same_prob(const Prob& a, const Prob& b) {
+  return a.rowptr == b.rowptr && a.colidx == b.colidx && a.vals == b.vals && a.b == b.b &&
+         a.x == b.x && a.inv == b.inv && a.st == b.st && a.part == b.part && a.hist == b.hist &&
+         a.n == b.n && a.lanes == b.lanes && a.ps == b.ps && a.m == b.m && a.maxit == b.maxit &&
+         a.hist_len == b.hist_len && a.jacobi == b.jacobi && a.nb == b.nb && a.nbs == b.nbs &&
+         a.V == b.V && a.H == b.H && a.cc == b.cc && a.ss == b.ss && a.grs == b.grs &&
+         a.lhh == b.lhh && a.R == b.R && a.RP == b.RP && a.Vb == b.Vb && a.T == b.T &&
+         a.S == b.S && a.P == b.P;
+}
+// Every member of Prob is compared by same_prob; a new member must be added
+// there (the size check below fails when one is added without updating it).
+static_assert(sizeof(Prob) == 9 * sizeof(void*) + 10 * sizeof(int) + 12 * sizeof(void*),
+              "Prob changed: update same_prob");
+
 Rounds* cached_rounds(int method, int device, const Prob& p) {
   static unsigned long long tick = 0;
   auto& c = round_cache();
   for (CacheEntry& e : c)
-    if (e.method == method && e.device == device && std::memcmp(&e.p, &p, sizeof(Prob)) == 0) {
+    if (e.method == method && e.device == device && same_prob(e.p, p)) {
       e.used = ++tick;
       return e.r.get();
     }

@@ -259,11 +259,18 @@ class Sparsity(base.Sparsity):
         handle = ctypes.c_void_p()
         nnz = ctypes.c_longlong()
         nel = (ctypes.c_longlong * self._nmaps)(*[m.iterset.size for m in self._rmaps])
+        rdims = nat.int_array([m.dim for m in self._rmaps])
+        wbytes = nat.lib.op2hip_sparsity_workspace(nrows, self._nmaps, nel, rdims)
+        if wbytes < 0:
+            raise RuntimeError("bad sparsity arguments")
+        # the workspace comes from torch's caching allocator: repeated builds
+        # reuse its blocks instead of allocating device memory
+        work = _dev_bytes(wbytes)
         nat.check(nat.lib.op2hip_sparsity_build(
-            rmult, cmult, nrows, self._nmaps, nat.ptr_array([_ptr(t) for t in rmaps]),
-            nat.int_array([m.dim for m in self._rmaps]),
+            rmult, cmult, nrows, self._nmaps, nat.ptr_array([_ptr(t) for t in rmaps]), rdims,
             nat.ptr_array([_ptr(t) for t in cmaps]),
-            nat.int_array([m.dim for m in self._cmaps]), nel, ctypes.c_void_p(_ptr(rowptr)),
+            nat.int_array([m.dim for m in self._cmaps]), nel, ctypes.c_void_p(_ptr(work)),
+            int(work.numel()), ctypes.c_void_p(_ptr(rowptr)),
             ctypes.byref(handle), ctypes.byref(nnz), ctypes.c_void_p(s)), 'sparsity build')
         colidx = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=device())
         nnz_rows = torch.empty(max(lsize, 1), dtype=torch.int32, device=device())
@@ -271,6 +278,7 @@ class Sparsity(base.Sparsity):
                                                ctypes.c_void_p(_ptr(colidx)),
                                                ctypes.c_void_p(_ptr(nnz_rows)),
                                                ctypes.c_void_p(s)), 'sparsity fill')
+        del work
         return rowptr, colidx, nnz_rows, int(nnz.value)
 
     def _row_layout(self):

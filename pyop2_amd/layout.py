@@ -32,6 +32,7 @@ import numpy as np
 
 from . import _native as nat
 from . import configuration as cfg
+from .exceptions import MapValueError
 
 
 class Layout(object):
@@ -132,6 +133,26 @@ def register_map(m):
         lst.append(m)
 
 
+def values(m):
+    """The values of Map ``m`` after a one-time check that every entry
+    names an element of its target Set.  Every device use of a map (the
+    layout decisions, which index permutations with its values, and the
+    upload) reads them through here: an entry out of range would otherwise
+    wrap to another element on the host (a negative numpy index) or gather
+    outside the target's device array."""
+    if m.values is not None and not getattr(m, '_range_checked', False):
+        v = np.asarray(m.values)
+        if v.size:
+            lo, hi = int(v.min()), int(v.max())
+            if lo < 0 or hi >= m.dataset.total_size:
+                bad = lo if lo < 0 else hi
+                raise MapValueError("Map %s has the value %d, outside its target Set %s of "
+                                    "size %d" % (m.name, bad, m.dataset.name,
+                                                 m.dataset.total_size))
+        m._range_checked = True
+    return m.values
+
+
 def decided(s):
     return _group(s).layout is not None
 
@@ -171,13 +192,13 @@ def _decide(g):
             m, col = pref
             T = of(m.dataset)
             if _group(m.dataset) is not g and not T.identity:
-                key = T.iperm[np.asarray(m.values)[:, col]]
+                key = T.iperm[np.asarray(values(m))[:, col]]
                 return Layout(np.argsort(key, kind='stable'))
     # 1. follow the first laid-out target set
     for m in maps_from:
         T = m.dataset
         if _group(T) is not g and decided(T) and not of(T).identity:
-            key = of(T).iperm[m.values].min(axis=1)
+            key = of(T).iperm[values(m)].min(axis=1)
             return Layout(np.argsort(key, kind='stable'))
     # 2. first touch from a laid-out source set
     for m in maps_to:
@@ -189,14 +210,14 @@ def _decide(g):
     #    through the largest map into the set
     if maps_to:
         m = max(maps_to, key=lambda x: x.values.size)
-        return Layout(adjacency_order(n, m.values, cfg.get('hip_layout_order', 'bisect')))
+        return Layout(adjacency_order(n, values(m), cfg.get('hip_layout_order', 'bisect')))
     # 4. an iteration set only: lay out its largest target first, then follow it
     if maps_from:
         m = max(maps_from, key=lambda x: x.dataset.total_size)
         if _group(m.dataset) is not g:
             T = of(m.dataset)
             if not T.identity:
-                key = T.iperm[m.values].min(axis=1)
+                key = T.iperm[values(m)].min(axis=1)
                 return Layout(np.argsort(key, kind='stable'))
     return IDENTITY
 
@@ -216,7 +237,7 @@ def adjacency_order(n, values, how='bisect', leaf=32):
 
 
 def _first_touch(n, m, src_order):
-    v = np.ascontiguousarray(m.values, dtype=np.int32)
+    v = np.ascontiguousarray(values(m), dtype=np.int32)
     perm = np.empty(n, dtype=np.int32)
     nat.check(nat.lib.op2hip_order_first_touch(
         n, len(src_order), v.ctypes.data_as(ctypes.c_void_p), v.shape[1],
@@ -230,8 +251,8 @@ def map_values(m):
     order, entries as positions in the target set's layout (int32)."""
     v = getattr(m, '_layout_vals', None)
     if v is None:
+        v = np.asarray(values(m))
         src, dst = of(m.iterset), of(m.dataset)
-        v = np.asarray(m.values)
         if not src.identity:
             v = v[src.perm]
         if not dst.identity:

@@ -136,3 +136,30 @@ def test_identical_maps_share_target_layout():
     ev = hip.Map(el, vnodes, 3, mesh.elem_node, 'ev')
     assert np.array_equal(layout.of(nodes).perm, layout.of(vnodes).perm)
     assert np.array_equal(layout.map_values(en), layout.map_values(ev))
+
+
+@pytest.mark.parametrize('bad', [-1, 5, 1 << 20])
+def test_map_values_out_of_range_raise(bad):
+    """A map entry outside its target Set raises MapValueError at its first
+    device use (layout decision or upload), before any host permutation
+    could wrap a negative entry to the last target or a gather could leave
+    the target's device array."""
+    from pyop2_amd.exceptions import MapValueError
+    src = hip.Set(4, 1, 'rc_src')
+    dst = hip.Set(5, 1, 'rc_dst')
+    vals = np.array([[0, 1], [1, 2], [2, 3], [3, 4]], dtype=np.int32)
+    vals[2, 1] = bad
+    m = hip.Map(src, dst, 2, vals, 'rc_map')
+    with pytest.raises(MapValueError):
+        layout.map_values(m)
+    with pytest.raises(MapValueError):
+        layout.of(dst)
+
+
+def test_map_values_in_range_pass():
+    src = hip.Set(3, 1, 'ok_src')
+    dst = hip.Set(4, 1, 'ok_dst')
+    m = hip.Map(src, dst, 2, np.array([[0, 3], [1, 2], [3, 0]], dtype=np.int32), 'ok_map')
+    dev = layout.map_values(m)
+    assert np.array_equal(layout.of(dst).perm[dev], m.values[layout.of(src).perm]
+                          if not layout.of(src).identity else m.values)
