@@ -125,6 +125,31 @@ int op2hip_owner_plan_build(int nelems, int ntargets, int nmaps, const int* cons
                             const int* dims, const int* idxs, int k, const int* elem_order,
                             void** plan_out);
 int op2hip_owner_plan_scalar(void* plan, int field, int64_t* out);
+/* The same owner plan built on the device by sort and scan (the form the
+ * backend uses; the host build above is its restatement in tests).  Also
+ * replaces pyop2/plan.pyx:68-456 for owner-computes loops.  maps: host array
+ * of device int32 map pointers (rows in the loop's element order); rank
+ * (device, nelems ints, may be NULL: the identity) orders each target's
+ * increments (the host build's elem_order, inverted).  work: a device buffer
+ * of op2hip_owner_plan_workspace() bytes, kept until fill or discard.  Phase
+ * 1 (dbuild) writes tp (device, ntargets + 1) and waits for the stream once
+ * to return scalars[6]: nactive, exec length, tgt length, max_exec, largest
+ * contribution count of a block, sum over blocks of elements executed.  The
+ * caller allocates the device arrays by these sizes; phase 2 (dfill) writes
+ * blk_t0, blk_t1, rng_lo, rng_n (nactive), ex_off, tg_off (nactive + 1),
+ * exec, tgt (tgt length + 1: one padding entry), tg_cb (each listed
+ * target's first contribution, tgt length + 2) and contrib (nelems * race
+ * columns entries, uint16 when c16 else int32), entry for entry the host
+ * build's arrays. */
+long long op2hip_owner_plan_workspace(int nelems, int ntargets, int nac, int k);
+int op2hip_owner_plan_dbuild(int nelems, int ntargets, int nmaps, const int* const* maps,
+                             const int* dims, const int* idxs, int k, const int* rank,
+                             void* work, long long work_bytes, int* tp, void** handle,
+                             long long* scalars, void* stream);
+int op2hip_owner_plan_dfill(void* handle, const int* tp, int* blk_t0, int* blk_t1, int* rng_lo,
+                            int* rng_n, int* ex_off, int* exec, int* tg_off, int* tgt,
+                            int* tg_cb, void* contrib, int c16, void* stream);
+int op2hip_owner_plan_ddiscard(void* handle);
 int op2hip_owner_plan_array(void* plan, int which, int* dst);
 int op2hip_owner_plan_destroy(void* plan);
 /* Scatter program of an owner-computes Mat assembly (no counterpart in the

@@ -88,6 +88,12 @@ _SIGS = {
     'op2hip_owner_plan_scalar': (c_int, [c_void, c_int, P(ctypes.c_int64)]),
     'op2hip_owner_plan_array': (c_int, [c_void, c_int, c_void]),
     'op2hip_owner_plan_destroy': (c_int, [c_void]),
+    'op2hip_owner_plan_workspace': (c_ll, [c_int, c_int, c_int, c_int]),
+    'op2hip_owner_plan_dbuild': (c_int, [c_int, c_int, c_int, P(c_void), P(c_int), P(c_int), c_int,
+                                         c_void, c_void, c_ll, c_void, P(c_void), P(c_ll),
+                                         c_void]),
+    'op2hip_owner_plan_dfill': (c_int, [c_void] + [c_void] * 11 + [c_int, c_void]),
+    'op2hip_owner_plan_ddiscard': (c_int, [c_void]),
     'op2hip_mat_program': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void, c_void,
                                    c_void, c_int,
                                    c_void, c_int, c_int, c_int, c_void, c_void, c_int, c_int,

@@ -4,6 +4,7 @@ Usage: ``python -m pyop2_amd.build``.  The library goes to
 ``pyop2_amd/lib/libop2hip.so``; it needs no GPU to build.
 """
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -14,7 +15,8 @@ LIBDIR = os.path.join(HERE, 'lib')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('OP2HIP_ARCH', 'gfx950')
 
-SOURCES = ['runtime.hip', 'plan.cpp', 'sparsity.hip', 'linalg.hip', 'cg_batch.hip', 'krylov.hip']
+SOURCES = ['runtime.hip', 'plan.cpp', 'plan_device.hip', 'sparsity.hip', 'linalg.hip',
+           'cg_batch.hip', 'krylov.hip']
 
 # Device code is built without packed FP32 instructions (some of their forms
 # have given wrong results on this hardware) and without FP contraction, so
@@ -38,7 +40,7 @@ def build(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(LIBDIR, 'obj')
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, os.path.splitext(src)[0] + '.o')
@@ -48,8 +50,11 @@ def build(verbose=False):
             cmd = [HIPCC, '-O3', '-std=c++17', '-fPIC', '-c', path, '-o', obj]
         if verbose:
             print(' '.join(cmd))
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(obj)
+    # the sources compile independently: one hipcc process each
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(cmds), 8)) as pool:
+        list(pool.map(_run, cmds))
     out = os.path.join(LIBDIR, 'libop2hip.so')
     tmp = out + '.tmp'
     cmd = [HIPCC, '--offload-arch=%s' % ARCH, '-shared', '-fPIC', '-o', tmp] + objs + \

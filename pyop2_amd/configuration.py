@@ -43,6 +43,7 @@ DEFAULTS = {
     'hip_owner_mat_lds': 0,       # LDS bytes an owner Mat block may use (0: a CU's share at
                                   # full occupancy, 20 KB for 256 threads)
     'hip_owner_waves': 0,         # waves per SIMD owner loops are compiled for (0: compiler's choice)
+    'hip_device_plan': True,      # owner plans built on the device (False: the host build)
     'hip_layout': True,           # locality renumbering of device storage
     'hip_layout_order': 'bisect',  # adjacency order of sets no map orders: 'bisect' or 'bfs'
     'hip_cache_dir': None,        # on-disk code-object cache (None: ~/.cache/pyop2_amd)

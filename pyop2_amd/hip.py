@@ -690,6 +690,7 @@ class OwnerPlan(object):
         # contribution codes per block (its owned targets' range of contrib)
         self.max_codes = int((self.tp[self.blk_t1] - self.tp[self.blk_t0]).max()) \
             if self.nactive else 0
+        self.sum_exec = int(self.nexec.sum()) if self.nactive else 0
         # device lists carry one padding entry (tgt 0, tg_cb its end), so a
         # kernel may load entry ng of a block unconditionally
         self._pad = {'tgt': 0, 'tg_cb': int(self.tp[-1])}
@@ -709,6 +710,10 @@ class OwnerPlan(object):
         """Contribution codes fit 16 bits (the device copy is uint16)."""
         return self.max_exec * self.nac <= 0xffff
 
+    @property
+    def exec_per_block(self):
+        return self.sum_exec / float(self.nactive) if self.nactive else 0.0
+
     def device_arrays(self):
         if self._dev is None:
             self._dev = {}
@@ -724,11 +729,101 @@ class OwnerPlan(object):
         return self._dev
 
 
+class DeviceOwnerPlan(object):
+    """The owner-computes plan built on the device
+    (csrc/plan_device.hip: op2hip_owner_plan_dbuild/dfill), entry for entry
+    the arrays of :class:`OwnerPlan`, which stays the host restatement the
+    tests compare it with.  ``cols`` are (map, idx) pairs of hip Maps (their
+    device values are read); ``rank`` (device int32, may be None) orders each
+    target's increments.  Host copies of the arrays are fetched on first
+    access (tests, the Mat scatter program)."""
+
+    def __init__(self, n, ntargets, cols, k, rank=None):
+        self.n, self.ntargets, self.k = n, ntargets, k
+        maps = [m._device_values() for m, _ in cols]
+        dims = nat.int_array([m.dim for m, _ in cols])
+        idxs = nat.int_array([i for _, i in cols])
+        nac = sum(m.dim if i < 0 else 1 for m, i in cols)
+        wbytes = nat.lib.op2hip_owner_plan_workspace(n, ntargets, nac, k)
+        if wbytes < 0:
+            raise RuntimeError("bad owner plan arguments")
+        work = _dev_bytes(wbytes)
+        tp = torch.empty(ntargets + 1, dtype=torch.int32, device=device())
+        h = ctypes.c_void_p()
+        sc = (ctypes.c_longlong * 6)()
+        s = stream()
+        nat.check(nat.lib.op2hip_owner_plan_dbuild(
+            n, ntargets, len(cols), nat.ptr_array([_ptr(t) for t in maps]), dims, idxs, k,
+            ctypes.c_void_p(_ptr(rank)), ctypes.c_void_p(_ptr(work)), int(work.numel()),
+            ctypes.c_void_p(_ptr(tp)), ctypes.byref(h), sc, ctypes.c_void_p(s)),
+            'device owner plan build')
+        self.nactive, nexec, ntgt, self.max_exec, self.max_codes, self.sum_exec = \
+            [int(x) for x in sc]
+        self.nac = nac
+        ninc = n * nac
+        i32 = lambda m: torch.empty(max(m, 1), dtype=torch.int32, device=device())
+        d = {'blk_t0': i32(self.nactive), 'blk_t1': i32(self.nactive),
+             'rng_lo': i32(self.nactive), 'rng_n': i32(self.nactive),
+             'ex_off': i32(self.nactive + 1), 'tg_off': i32(self.nactive + 1),
+             'exec': i32(nexec), 'tgt': i32(ntgt + 1), 'tg_cb': i32(ntgt + 2), 'tp': tp,
+             'contrib': torch.empty(max(ninc, 1), dtype=torch.int16 if self.c16 else torch.int32,
+                                    device=device())}
+        P = lambda name: ctypes.c_void_p(_ptr(d[name]))
+        nat.check(nat.lib.op2hip_owner_plan_dfill(
+            h, P('tp'), P('blk_t0'), P('blk_t1'), P('rng_lo'), P('rng_n'), P('ex_off'), P('exec'),
+            P('tg_off'), P('tgt'), P('tg_cb'), P('contrib'), int(self.c16), ctypes.c_void_p(s)),
+            'device owner plan fill')
+        del work
+        self._sizes = {'blk_t0': self.nactive, 'blk_t1': self.nactive, 'rng_lo': self.nactive,
+                       'rng_n': self.nactive, 'ex_off': self.nactive + 1,
+                       'tg_off': self.nactive + 1, 'exec': nexec, 'tgt': ntgt,
+                       'tg_cb': ntgt + 1, 'tp': ntargets + 1, 'contrib': ninc}
+        self._dev = d
+        self._host = {}
+
+    @property
+    def c16(self):
+        return self.max_exec * self.nac <= 0xffff
+
+    @property
+    def exec_per_block(self):
+        return self.sum_exec / float(self.nactive) if self.nactive else 0.0
+
+    def device_arrays(self):
+        return self._dev
+
+    def __getattr__(self, name):
+        sizes = self.__dict__.get('_sizes')
+        if sizes is None or name not in sizes:
+            raise AttributeError(name)
+        a = self._host.get(name)
+        if a is None:
+            t = self._dev[name][:sizes[name]]
+            a = t.cpu().numpy()
+            if name == 'contrib' and self.c16:
+                a = a.view(np.uint16)
+            a = a.astype(np.int32)
+            self._host[name] = a
+        return a
+
+    def block_elements(self, b):
+        return OwnerPlan.block_elements(self, b)
+
+
 _owner_cache = {}
 OWNER_LDS_LIMIT = 64 * 1024
 # MI355X (gfx950) compute unit: 160 KB of LDS, 4 SIMDs of at most 8 waves
 LDS_PER_CU = 160 * 1024
 WAVES_PER_CU = 32
+
+
+def _device_perm(lay):
+    """A layout's permutation as a device int32 array (cached)."""
+    d = getattr(lay, '_dev_perm', None)
+    if d is None:
+        d = torch.from_numpy(np.ascontiguousarray(lay.perm, dtype=np.int32)).to(device())
+        lay._dev_perm = d
+    return d
 
 
 def mat_program_host(plan, cmap, e1, nnodes, rowptr, colidx, rd, cd):
@@ -805,13 +900,20 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None,
     if key in _owner_cache:
         return _owner_cache[key]
     lay = layout.of(iterset)
-    # increments are summed in user element order (the sequential order):
-    # the device position of user element i is iperm[i]
-    order = None if lay.identity else lay.iperm[:n]
-    vals = [(m._layout_values()[:n], m.dim, i) for m, i in cols]
+    if cfg.get('hip_device_plan', True):
+        # increments are summed in user element order (the sequential
+        # order): device element e is user element perm[e]
+        rank = None if lay.identity else _device_perm(lay)
 
-    def build(kk):
-        return OwnerPlan(n, target.total_size, vals, kk, order)
+        def build(kk):
+            return DeviceOwnerPlan(n, target.total_size, cols, kk, rank)
+    else:
+        # the host build: the device position of user element i is iperm[i]
+        order = None if lay.identity else lay.iperm[:n]
+        vals = [(m._layout_values()[:n], m.dim, i) for m, i in cols]
+
+        def build(kk):
+            return OwnerPlan(n, target.total_size, vals, kk, order)
 
     limit = cfg.get('hip_owner_lds') or min(OWNER_LDS_LIMIT, _state.get('lds_per_block') or 65536)
     if lds_limit:
@@ -824,7 +926,7 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None,
         p = build(kk)
         if p.nactive:
             # elements run and LDS bytes per owned target, from the trial
-            per_target = (p.nexec.sum() / float(p.nactive)) / kk
+            per_target = p.exec_per_block / kk
             lds_per_target = lds_of(p) / float(kk)
             fill = cfg.get('hip_owner_fill') or 1.0
             kk = min(1 << 16, max(1, int(fill * block / max(per_target, 1e-9))))
@@ -836,7 +938,7 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None,
     if lds_of(p) > limit:
         _owner_cache[key] = None
         return None
-    if p.nactive and p.nexec.sum() / float(p.nactive) < 32 and not k_cfg:
+    if p.nactive and p.exec_per_block < 32 and not k_cfg:
         # blocks would run too few elements to keep a workgroup busy
         _owner_cache[key] = None
         return None

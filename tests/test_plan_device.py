@@ -1,0 +1,98 @@
+"""The owner-computes plan built on the device against the host build
+(gpu): every array entry for entry (tp, contrib, blk_t0/blk_t1, rng_lo/rng_n,
+ex_off/exec, tg_off/tgt, tg_cb) and every scalar, on random maps, airfoil
+edges, primary runs, element orders given as ranks, all-column (Mat) race
+maps, a fan whose hub target has thousands of increments (the long-segment
+sort) and blocks of more than 4096 incidences (the block pass in global
+memory).  The host build is itself checked against the plan invariants in
+test_plan.py (conflict-freedom, each increment once, sequential order)."""
+
+import numpy as np
+import pytest
+
+from pyop2_amd.workloads import meshes
+
+pytestmark = pytest.mark.gpu
+
+ARRAYS = ('blk_t0', 'blk_t1', 'rng_lo', 'rng_n', 'ex_off', 'exec', 'tg_off', 'tgt', 'tp',
+          'contrib', 'tg_cb')
+
+
+def build_both(op2, vals, ntargets, colspec, k, order=None):
+    """colspec: [(values, idx)]; order: user element order (elem_order of
+    the host build) or None."""
+    from pyop2_amd import hip
+    n = vals[0].shape[0]
+    src = op2.Set(n, 1)
+    dst = op2.Set(ntargets, 1)
+    from pyop2_amd import layout
+    cols = []
+    for v, idx in colspec:
+        m = op2.Map(src, dst, v.shape[1], v)
+        cols.append((m, idx))
+    # identity device layouts: the device values are the user values
+    lay_src, lay_dst = layout._group(src), layout._group(dst)
+    lay_src.layout = layout.IDENTITY
+    lay_dst.layout = layout.IDENTITY
+    rank = None
+    if order is not None:
+        r = np.empty(n, dtype=np.int32)
+        r[order] = np.arange(n, dtype=np.int32)
+        rank = hip.torch.from_numpy(r).to(hip.device())
+    d = hip.DeviceOwnerPlan(n, ntargets, cols, k, rank)
+    h = hip.OwnerPlan(n, ntargets, [(v, v.shape[1], i) for v, i in colspec], k,
+                      None if order is None else np.ascontiguousarray(order, dtype=np.int32))
+    return d, h
+
+
+def same(d, h):
+    for name in ('nactive', 'max_exec', 'nac', 'max_codes', 'sum_exec'):
+        assert getattr(d, name) == getattr(h, name), name
+    for name in ARRAYS:
+        a, b = np.asarray(getattr(d, name)), np.asarray(getattr(h, name))
+        assert a.shape == b.shape and np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize('seed', range(4))
+@pytest.mark.parametrize('k', [1, 7, 64, 300])
+def test_random_maps(hip, seed, k):
+    rng = np.random.default_rng(seed)
+    n, nt, dim = 3000, 900, 3
+    v = rng.integers(0, nt, size=(n, dim)).astype(np.int32)
+    spec = [(v, -1)] if seed % 2 else [(v, 0), (v, dim - 1)]
+    order = rng.permutation(n) if seed >= 2 else None
+    same(*build_both(hip, [v], nt, spec, k, order))
+
+
+def test_airfoil_edges_with_runs(hip):
+    m = meshes.QuadMesh(60, 30, seed=5)
+    order = np.argsort(m.pecell[:, 0], kind='stable')
+    pec = np.ascontiguousarray(m.pecell[order])
+    d, h = build_both(hip, [pec], m.ncell, [(pec, 0), (pec, 1)], 64)
+    same(d, h)
+    assert (d.rng_n > 0).all()
+
+
+def test_mat_columns_user_order(hip):
+    mesh = meshes.TetMesh(8, seed=3)
+    en = mesh.elem_node.astype(np.int32)
+    order = np.random.default_rng(1).permutation(en.shape[0])
+    same(*build_both(hip, [en], mesh.nvert, [(en, -1)], 13, order))
+
+
+def test_fan_and_large_blocks(hip):
+    """A hub target named by 5000 elements (a segment beyond the wavefront
+    sort) and blocks of 2000 targets (more incidences than the LDS pass
+    holds)."""
+    n = 5000
+    k_ = np.arange(1, n + 1)
+    fan = np.stack([np.zeros(n, dtype=np.int64), k_, k_ + 1], axis=1).astype(np.int32)
+    order = np.random.default_rng(4).permutation(n)
+    same(*build_both(hip, [fan], n + 2, [(fan, -1)], 5, order))
+    same(*build_both(hip, [fan], n + 2, [(fan, -1)], 2000, order))
+
+
+def test_empty_targets_and_elements(hip):
+    v = np.array([[3, 3], [3, 9], [9, 3]], dtype=np.int32)
+    same(*build_both(hip, [v], 40, [(v, -1)], 4))
+    same(*build_both(hip, [v], 40, [(v, 1)], 64))
