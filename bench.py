@@ -43,11 +43,9 @@ def parse():
     p.add_argument('--vector-maps', type=int, default=0,
                    help='drive the step in the vector-map form (1) instead of the scalar one')
     p.add_argument('--ny', type=int, default=600)
+    p.add_argument('--soa', type=int, default=0,
+                   help='store q, q_old and res as structure-of-arrays Dats (1)')
     p.add_argument('--graph', type=int, default=1, help='replay each step as a hipGraph')
-    p.add_argument('--fuse', type=int, default=0,
-                   help='fuse consecutive owner loops into one Dat in the captured step')
-    p.add_argument('--chain-waves', type=int, default=-1,
-                   help='waves per SIMD a fused kernel is budgeted for (-1: default, 0: none)')
     p.add_argument('--cold', type=int, default=1,
                    help='time res_calc and adt_calc cache-cold too (0: skip)')
     p.add_argument('--cpu-seconds', type=float, default=15.0,
@@ -138,8 +136,6 @@ def main():
         opts['hip_owner_prefetch'] = args.owner_prefetch
     if args.owner_waves >= 0:
         opts['hip_owner_waves'] = args.owner_waves
-    if args.chain_waves >= 0:
-        opts['hip_chain_waves'] = args.chain_waves
     if args.owner_stage >= 0:
         opts['hip_owner_stage'] = bool(args.owner_stage)
     if args.autotune:
@@ -156,7 +152,8 @@ def main():
         opts['hip_owner_targets'] = args.owner_targets
     op2.init(**opts)
     mesh = meshes.QuadMesh(args.nx, args.ny, seed=0)
-    A = (airfoil.AirfoilVector if args.vector_maps else airfoil.Airfoil)(op2, mesh)
+    A = (airfoil.AirfoilVector if args.vector_maps else airfoil.Airfoil)(op2, mesh,
+                                                                         soa=bool(args.soa))
     ub = airfoil.unique_bytes(mesh)
     step_bytes = ub['save_soln'] + 2 * (ub['adt_calc'] + ub['res_calc'] + ub['bres_calc'] +
                                         ub['update'])
@@ -168,7 +165,7 @@ def main():
     step = A.iteration
     graph = None
     if args.graph:
-        graph = hip.capture(A.iteration, fuse=bool(args.fuse))
+        graph = hip.capture(A.iteration)
         graph.replay()
         torch.cuda.synchronize()
         step = graph.replay
@@ -195,7 +192,7 @@ def main():
     def hot_avg(loop, nrep=10):
         # one loop alone: a graph of `nrep` back-to-back calls, HIP events
         # on the stream around its replays (no host time in between)
-        g = hip.capture(lambda: [loop() for _ in range(nrep)], fuse=False)
+        g = hip.capture(lambda: [loop() for _ in range(nrep)])
         g.replay()
         torch.cuda.synchronize()
         a, b = mkev(), mkev()
@@ -214,7 +211,7 @@ def main():
         # L2s and the 256 MB Infinity Cache together), so its inputs come
         # from HBM; the read leaves the caches holding clean lines, so the
         # loop does not pay for writing back the flush's own data
-        g = hip.capture(loop, fuse=False)
+        g = hip.capture(loop)
         g.replay()
         torch.cuda.synchronize()
         evs = [(mkev(), mkev()) for _ in range(nrep)]
@@ -270,8 +267,8 @@ def main():
                    'unique_bytes_per_step': int(step_bytes),
                    'pct_of_hbm_peak': round(100.0 * value / HBM_PEAK_GBS, 2),
                    'graph': bool(args.graph),
-                   'fused_loops': bool(args.graph and args.fuse),
                    'vector_maps': bool(args.vector_maps),
+                   'soa': bool(args.soa),
                    'autotune': _tuned() if args.autotune else False,
                    'partition_size': hip.cfg.get('hip_partition_size') or 'auto'},
         'roofline': {'bound': 'hbm', 'kernel': 'op2_res_calc_owner (one res_calc par_loop)',

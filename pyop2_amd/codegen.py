@@ -78,6 +78,23 @@ class ArgInfo(object):
                 self.idx = arg.idx
         self.race = (self.kind == 'mat') or \
             (self.kind in ('ind', 'vec', 'itspace') and self.access is INC)
+        # structure-of-arrays device storage (reference: device.py:153-174,
+        # cuda_direct_loop.jinja2:80-82): component k of row r at
+        # k * stride + r, stride the Dat's rows on the device.  The kernel
+        # sees a private copy of each row (OP2_STRIDE(a, k) is a[k]), so
+        # the loads and stores of consecutive elements are contiguous per
+        # component.
+        self.soa = bool(self.kind in ('direct', 'ind', 'vec', 'itspace') and d.soa and d.cdim > 1)
+        self.stride = d.dataset.total_size if self.soa else 0
+        if self.soa and self.kind in ('vec', 'itspace') and self.access not in (READ, INC):
+            raise NotImplementedError("structure-of-arrays Dats are written through vector "
+                                      "and iteration-space maps only by INC")
+
+    def at(self, row, k):
+        """Index expression of component ``k`` of device row ``row``."""
+        if self.soa:
+            return '(op2_i64)(%s) * %d + (op2_i64)(%s)' % (k, self.stride, row)
+        return '(op2_i64)(%s) * %d + (%s)' % (row, self.cdim, k)
 
     # parameter names
     @property
@@ -309,6 +326,17 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                     call_args.append('(prim ? red%d : %s)' % (p, n('junk')))
                 else:
                     call_args.append('red%d' % p)
+            elif a.kind == 'direct' and a.soa:
+                # a private row, loaded (unless written only) and stored back
+                decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
+                if a.access is not WRITE:
+                    (loads if local else compute).append(
+                        'for (int k = 0; k < %d; ++k) %s[k] = d%d[%s];'
+                        % (a.cdim, n('rl'), p, a.at(E, 'k')))
+                if a.access is not READ:
+                    post.append('for (int k = 0; k < %d; ++k) d%d[%s] = %s[k];'
+                                % (a.cdim, p, a.at(E, 'k'), n('rl')))
+                call_args.append(n('rl'))
             elif a.kind == 'direct':
                 if self.mode == 'stream' and self._staged(a):
                     call_args.append('st%d + threadIdx.x * %d' % (p, self._pitch(a)))
@@ -328,10 +356,25 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                 else:
                     call_args.append('d%d + %s * %d' % (p, E, a.cdim))
             elif a.kind == 'ind':
+                row = '(op2_i64)m%d[%s * %d + %d]' % (self._m(p), E, a.mdim, a.idx)
                 if a.race:
                     decls.append('%s %s[%d];' % (t, n('inc'), a.cdim))
                     decls.append('for (int k = 0; k < %d; ++k) %s[k] = (%s)0;' % (a.cdim, n('inc'), t))
                     call_args.append(n('inc'))
+                elif a.soa:
+                    decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
+                    if a.access is not READ:
+                        decls.append('op2_i64 %s = 0;' % n('rw'))
+                        compute.append('%s = %s;' % (n('rw'), row))
+                        row = n('rw')
+                    if a.access is not WRITE:
+                        (loads if local else compute).append(
+                            '{ const op2_i64 tt = %s; for (int k = 0; k < %d; ++k) %s[k] = d%d[%s]; }'
+                            % (row, a.cdim, n('rl'), p, a.at('tt', 'k')))
+                    if a.access is not READ:
+                        post.append('for (int k = 0; k < %d; ++k) d%d[%s] = %s[k];'
+                                    % (a.cdim, p, a.at(row, 'k'), n('rl')))
+                    call_args.append(n('rl'))
                 elif local and p in self.owner_stage:
                     # rows of the block's owned range come from LDS
                     decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
@@ -357,6 +400,13 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                     decls.append('for (int j = 0; j < %d; ++j) { %s[j] = %s[j]; '
                                  'for (int k = 0; k < %d; ++k) %s[j][k] = (%s)0; }'
                                  % (a.mdim, n('v'), n('inc'), a.cdim, n('inc'), t))
+                elif a.soa:
+                    decls.append('%s %s[%d][%d];' % (t, n('rl'), a.mdim, a.cdim))
+                    (loads if local else compute).append(
+                        'for (int j = 0; j < %d; ++j) { const op2_i64 tt = m%d[%s * %d + j]; '
+                        '%s[j] = %s[j]; for (int k = 0; k < %d; ++k) %s[j][k] = d%d[%s]; }'
+                        % (a.mdim, self._m(p), E, a.mdim, n('v'), n('rl'), a.cdim, n('rl'), p,
+                           a.at('tt', 'k')))
                 elif local:
                     decls.append('%s %s[%d][%d];' % (t, n('rl'), a.mdim, a.cdim))
                     loads.append('for (int j = 0; j < %d; ++j) { const %s* src = d%d + '
@@ -376,6 +426,14 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                     decls.append('for (int j = 0; j < %d; ++j) for (int k = 0; k < %d; ++k) '
                                  '%s[j][k] = (%s)0;' % (a.extent, a.cdim, n('inc'), t))
                     call_args.append('%s[%s]' % (n('inc'), ivar))
+                elif a.soa:
+                    decls.append('%s %s[%d][%d];' % (t, n('rl'), a.extent, a.cdim))
+                    compute.append('for (int j = 0; j < %d; ++j) { const op2_i64 tt = '
+                                   'm%d[%s * %d + j]; for (int k = 0; k < %d; ++k) '
+                                   '%s[j][k] = d%d[%s]; }'
+                                   % (a.extent, self._m(p), E, a.mdim, a.cdim, n('rl'), p,
+                                      a.at('tt', 'k')))
+                    call_args.append('%s[%s]' % (n('rl'), ivar))
                 else:
                     call_args.append('d%d + (op2_i64)m%d[%s * %d + %s] * %d'
                                      % (p, self._m(p), E, a.mdim, ivar, a.cdim))
@@ -427,19 +485,16 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             if not a.race:
                 continue
             if a.kind == 'ind':
-                lines.append('{ %s* t = d%d + (op2_i64)m%d[(op2_i64)e * %d + %d] * %d;'
-                             % (a.ctype, p, p, a.mdim, a.idx, a.cdim))
-                lines.append('  for (int k = 0; k < %d; ++k) t[k] += inc%d[k]; }' % (a.cdim, p))
-            elif a.kind == 'vec':
-                lines.append('for (int j = 0; j < %d; ++j) { %s* t = d%d + '
-                             '(op2_i64)m%d[(op2_i64)e * %d + j] * %d;'
-                             % (a.mdim, a.ctype, p, p, a.mdim, a.cdim))
-                lines.append('  for (int k = 0; k < %d; ++k) t[k] += inc%d[j][k]; }' % (a.cdim, p))
-            elif a.kind == 'itspace':
-                lines.append('for (int j = 0; j < %d; ++j) { %s* t = d%d + '
-                             '(op2_i64)m%d[(op2_i64)e * %d + j] * %d;'
-                             % (a.extent, a.ctype, p, p, a.mdim, a.cdim))
-                lines.append('  for (int k = 0; k < %d; ++k) t[k] += inc%d[j][k]; }' % (a.cdim, p))
+                lines.append('{ const op2_i64 tt = m%d[(op2_i64)e * %d + %d];'
+                             % (p, a.mdim, a.idx))
+                lines.append('  for (int k = 0; k < %d; ++k) d%d[%s] += inc%d[k]; }'
+                             % (a.cdim, p, a.at('tt', 'k'), p))
+            elif a.kind in ('vec', 'itspace'):
+                lines.append('for (int j = 0; j < %d; ++j) { const op2_i64 tt = '
+                             'm%d[(op2_i64)e * %d + j];'
+                             % (a.mdim if a.kind == 'vec' else a.extent, p, a.mdim))
+                lines.append('  for (int k = 0; k < %d; ++k) d%d[%s] += inc%d[j][k]; }'
+                             % (a.cdim, p, a.at('tt', 'k'), p))
             elif a.kind == 'mat':
                 e0 = ext[0] if len(ext) > 0 else 1
                 e1 = ext[1] if len(ext) > 1 else 1
@@ -473,7 +528,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         contiguous instead of one strided row per thread (strided row
         stores cost partial-line writes).  Read-only ones are staged only
         when ``stage_reads`` is set."""
-        if a.kind != 'direct' or a.cdim == 1:
+        if a.kind != 'direct' or a.cdim == 1 or a.soa:
             return False
         if a.access is READ:
             return self.stage_reads
@@ -483,7 +538,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         """Direct Dats with several components that the kernel writes,
         when not staged through LDS: the row lives in registers and moves
         with the widest aligned vector loads and stores (whole rows)."""
-        return (self.mode == 'stream' and a.kind == 'direct' and a.cdim > 1 and
+        return (self.mode == 'stream' and a.kind == 'direct' and a.cdim > 1 and not a.soa and
                 a.access is not READ and not self._staged(a))
 
     def _row_types(self):
@@ -615,7 +670,8 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                 store.append('for (int j = 0; j < %d; ++j) for (int k = 0; k < %d; ++k) '
                              'op2_lds[((%d + j) * %d + k) * pitch + l] = inc%d[j][k];'
                              % (n, cdim, ac0, cdim, a.pos))
-        sub = {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos, 'nac': nac, 'cmax': OWNER_CMAX}
+        sub = {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos, 'nac': nac, 'cmax': OWNER_CMAX,
+               'at_f': tgt.at('f_row', 'k'), 'at_w': tgt.at('w_t', 'k')}
         codes = '''#pragma unroll
     for (int u = 0; u < %(cmax)d; ++u) w_code[u] = u < w_nc ? OP2_CONTRIB(w_cb + u) : 0;''' % sub
         out = ['  extern __shared__ %s op2_lds[];' % t,
@@ -657,7 +713,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   const int f_row = dense ? (jf < ng ? t0 + jf : 0) : f_t;
   %(t)s f_old[%(cdim)d];
 #pragma unroll
-  for (int k = 0; k < %(cdim)d; ++k) f_old[k] = d%(pos)d[(op2_i64)f_row * %(cdim)d + k];''' % sub)
+  for (int k = 0; k < %(cdim)d; ++k) f_old[k] = d%(pos)d[%(at_f)s];''' % sub)
         out += ['''  for (int l = threadIdx.x; l < nex; l += %d) {
     const unsigned code = l < rn ? (unsigned)(rlo + l) : (unsigned)p_exec[ex0 + l - rn];
     const int e = (int)(code & 0x7fffffffu);
@@ -691,11 +747,11 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             out.append('''    %(t)s acc[%(cdim)d];
 #pragma unroll
     for (int k = 0; k < %(cdim)d; ++k)
-      acc[k] = j == (int)threadIdx.x ? f_old[k] : d%(pos)d[(op2_i64)w_t * %(cdim)d + k];''' % sub)
+      acc[k] = j == (int)threadIdx.x ? f_old[k] : d%(pos)d[%(at_w)s];''' % sub)
         else:
             out.append('''    %(t)s acc[%(cdim)d];
 #pragma unroll
-    for (int k = 0; k < %(cdim)d; ++k) acc[k] = d%(pos)d[(op2_i64)w_t * %(cdim)d + k];''' % sub)
+    for (int k = 0; k < %(cdim)d; ++k) acc[k] = d%(pos)d[%(at_w)s];''' % sub)
         out.append('''#pragma unroll
     for (int u = 0; u < %(cmax)d; ++u) {
       if (u < w_nc) {
@@ -711,7 +767,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
       for (int k = 0; k < %(cdim)d; ++k) acc[k] += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
     }
 #pragma unroll
-    for (int k = 0; k < %(cdim)d; ++k) d%(pos)d[(op2_i64)w_t * %(cdim)d + k] = acc[k];
+    for (int k = 0; k < %(cdim)d; ++k) d%(pos)d[%(at_w)s] = acc[k];
   }''' % sub)
         out.append(ind(self._red_finish('b'), 2))
         return out
@@ -992,175 +1048,3 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             lines.append('  }')
         lines.append('}')
         return '\n'.join(lines)
-
-
-class ChainSource(LoopSource):
-    """One owner-computes kernel running several consecutive loops that
-    increment the same Dat (and touch it no other way): a block runs the
-    elements of every loop that name its targets, keeps their increments in
-    LDS, and sums each owned target's increments loop by loop, each loop's
-    in ascending user element order, which is the order the loops reach
-    the target one after the other.  ``parts`` is a list of (kernel, args);
-    the loops have no iteration space, Mats or Global reductions."""
-
-    def __init__(self, parts, consts, block_size, prefetch=1):
-        args = [a for _, pargs in parts for a in pargs]
-        LoopSource.__init__(self, parts[0][0], (), args, consts, 'owner', block_size,
-                            prefetch=prefetch)
-        self.parts = []
-        off = 0
-        for kernel, pargs in parts:
-            self.parts.append((kernel, self.infos[off:off + len(pargs)]))
-            off += len(pargs)
-        self.entry = 'op2_%s_chain' % '_'.join(k.name for k, _ in parts)
-        self.reduce_entry = None
-        # optional occupancy request (waves per SIMD) for the register budget
-        self.min_waves = 0
-
-    def part_columns(self, j):
-        """owner_columns of part ``j`` alone (its race columns from 0)."""
-        out, ac = [], 0
-        for a in self.parts[j][1]:
-            if not a.race:
-                continue
-            n = 1 if a.kind == 'ind' else (a.mdim if a.kind == 'vec' else a.extent)
-            out.append((a, ac, n))
-            ac += n
-        return out
-
-    def nac(self):
-        return max(sum(n for _, _, n in self.part_columns(j)) for j in range(len(self.parts)))
-
-    def owner_slot_bytes(self):
-        a = self.part_columns(0)[0][0]
-        return self.nac() * a.cdim * np.dtype(a.arg.data.dtype).itemsize
-
-    def owner_lds_bytes(self, k, max_exec, max_codes):
-        return max_exec * self.owner_slot_bytes()
-
-    def param_roles(self):
-        roles = [('start', None), ('count', None), ('own_t0', None), ('own_t1', None)]
-        for j in range(len(self.parts)):
-            roles += [('own_rlo%d' % j, None), ('own_rn%d' % j, None), ('own_exoff%d' % j, None),
-                      ('own_exec%d' % j, None)]
-        roles += [('own_tgoff', None), ('own_tgt', None), ('own_tgcb', None), ('own_tp', None),
-                  ('own_contrib', None), ('own_c16', None), ('own_pitch', None)]
-        for a in self.infos:
-            for _, role in a.params():
-                roles.append((role, a.pos))
-        return roles
-
-    def _params_decl(self):
-        decl = ['op2_i64 start', 'op2_i64 count', 'const int* __restrict__ p_t0',
-                'const int* __restrict__ p_t1']
-        for j in range(len(self.parts)):
-            decl += ['const int* __restrict__ p_rlo%d' % j, 'const int* __restrict__ p_rn%d' % j,
-                     'const int* __restrict__ p_exoff%d' % j, 'const int* __restrict__ p_exec%d' % j]
-        decl += ['const int* __restrict__ p_tgoff', 'const int* __restrict__ p_tgt',
-                 'const int* __restrict__ p_tgcb', 'const int* __restrict__ p_tp',
-                 'const void* __restrict__ p_contrib', 'op2_i64 own_c16', 'op2_i64 lds_pitch']
-        for a in self.infos:
-            decl += [d for d, _ in a.params()]
-        return decl
-
-    def source(self):
-        ind = lambda lines, n: '\n'.join(' ' * n + l for l in lines)
-        bs = self.bs
-        out = [self.PRELUDE, self._const_decls(), self._row_types()]
-        seen = set()
-        for kernel, _ in self.parts:
-            if kernel.name in seen:
-                continue
-            seen.add(kernel.name)
-            out += ['#pragma clang force_cuda_host_device begin', 'inline', kernel.code,
-                    '#pragma clang force_cuda_host_device end', '']
-        bounds = '%d, %d' % (bs, self.min_waves) if self.min_waves else '%d' % bs
-        out.append('extern "C" __global__ void __launch_bounds__(%s) %s(\n    %s) {'
-                   % (bounds, self.entry, ',\n    '.join(self._params_decl())))
-        tgt = self.part_columns(0)[0][0]
-        t, cdim = tgt.ctype, tgt.cdim
-        nac = self.nac()
-        sub = {'cdim': cdim, 'bs': bs, 't': t, 'pos': tgt.pos, 'nac': nac, 'cmax': OWNER_CMAX}
-        out.append('''  extern __shared__ %(t)s op2_lds[];
-  const int nb = (int)count, bid = blockIdx.x;
-  const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
-  const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
-  const int g0 = p_tgoff[b], ng = p_tgoff[b + 1] - g0;
-  const int t0 = p_t0[b];
-  const bool dense = ng == p_t1[b] - t0;
-  const int pitch = (int)lds_pitch;
-  // this thread's first target and its contribution range, loaded while
-  // the elements run (index clamped to the padding entry of the list)
-  const int jf = (int)threadIdx.x < ng ? (int)threadIdx.x : ng;
-  const int f_t = dense ? t0 + jf : p_tgt[g0 + jf];
-  const int f_cb = p_tp[f_t], f_ce = p_tp[f_t + 1];''' % sub)
-        # slots: part 0's elements first, then part 1's, ...
-        out.append('  int nex = 0;')
-        for j in range(len(self.parts)):
-            out.append('  const int rlo%d = p_rlo%d[b], rn%d = p_rn%d[b], ex%d = p_exoff%d[b];'
-                       % (j, j, j, j, j, j))
-            out.append('  const int sb%d = nex, ne%d = rn%d + p_exoff%d[b + 1] - ex%d;'
-                       % (j, j, j, j, j))
-            out.append('  nex += ne%d;' % j)
-        out.append('  for (int l = threadIdx.x; l < nex; l += %d) {' % bs)
-        for j, (kernel, infos) in enumerate(self.parts):
-            decls, compute = self._element_body(infos=infos, kernel=kernel)
-            store = []
-            for a, ac0, n in self.part_columns(j):
-                if a.kind == 'ind':
-                    store.append('for (int k = 0; k < %d; ++k) op2_lds[(%d + k) * pitch + l] = '
-                                 'inc%d[k];' % (cdim, ac0 * cdim, a.pos))
-                else:
-                    store.append('for (int j = 0; j < %d; ++j) for (int k = 0; k < %d; ++k) '
-                                 'op2_lds[((%d + j) * %d + k) * pitch + l] = inc%d[j][k];'
-                                 % (n, cdim, ac0, cdim, a.pos))
-            if j == 0:
-                out.append('    if (l < sb0 + ne0) {')
-            elif j < len(self.parts) - 1:
-                out.append('    else if (l < sb%d + ne%d) {' % (j, j))
-            else:
-                out.append('    else {')
-            out.append('      const int lj = l - sb%d;' % j)
-            out.append('      const unsigned code = lj < rn%d ? (unsigned)(rlo%d + lj) : '
-                       '(unsigned)p_exec%d[ex%d + lj - rn%d];' % (j, j, j, j, j))
-            out.append('      const int e = (int)(code & 0x7fffffffu);')
-            out.append(ind(decls + compute + store, 6))
-            out.append('    }')
-        out.append('  }')
-        out.append('  __syncthreads();')
-        out.append('''  for (int j = threadIdx.x; j < ng; j += %(bs)d) {
-    int w_t, w_cb, w_nc;
-    if (j == (int)threadIdx.x) {
-      w_t = f_t;
-      w_cb = f_cb;
-      w_nc = f_ce - f_cb;
-    } else {
-      w_t = dense ? t0 + j : p_tgt[g0 + j];
-      w_cb = p_tp[w_t];
-      w_nc = p_tp[w_t + 1] - w_cb;
-    }
-    int w_code[%(cmax)d];
-#pragma unroll
-    for (int u = 0; u < %(cmax)d; ++u) w_code[u] = u < w_nc ? OP2_CONTRIB(w_cb + u) : 0;
-    %(t)s acc[%(cdim)d];
-#pragma unroll
-    for (int k = 0; k < %(cdim)d; ++k) acc[k] = d%(pos)d[(op2_i64)w_t * %(cdim)d + k];
-#pragma unroll
-    for (int u = 0; u < %(cmax)d; ++u) {
-      if (u < w_nc) {
-        const int slot = w_code[u] / %(nac)d, ac = w_code[u] - slot * %(nac)d;
-#pragma unroll
-        for (int k = 0; k < %(cdim)d; ++k) acc[k] += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
-      }
-    }
-    for (int c = %(cmax)d; c < w_nc; ++c) {
-      const int cc = OP2_CONTRIB(w_cb + c);
-      const int slot = cc / %(nac)d, ac = cc - slot * %(nac)d;
-#pragma unroll
-      for (int k = 0; k < %(cdim)d; ++k) acc[k] += op2_lds[(ac * %(cdim)d + k) * pitch + slot];
-    }
-#pragma unroll
-    for (int k = 0; k < %(cdim)d; ++k) d%(pos)d[(op2_i64)w_t * %(cdim)d + k] = acc[k];
-  }
-}''' % sub)
-        return '\n'.join(out)

@@ -36,8 +36,6 @@ DEFAULTS = {
     'hip_stage_reads': False,     # stage read-only multi-component direct Dats through LDS
     'hip_stage_writes': True,     # stage written ones through LDS (else whole-row vector stores)
     'hip_owner_stage': False,     # owner blocks stage their owned rows of gathered Dats in LDS
-    'hip_chain': True,            # fuse consecutive owner loops into one Dat in fusing captures
-    'hip_chain_waves': 8,         # waves per SIMD a chain kernel's registers are budgeted for
     'hip_owner_prefetch': 1,      # owner write-back loads issued before the elements run:
                                   # 0 none, 1 the first target's lookups, 2 also its old value
     'hip_owner_mat_lds': 0,       # LDS bytes an owner Mat block may use (0: a CU's share at

@@ -36,7 +36,7 @@ from . import layout
 from . import autotune
 from .base import (READ, WRITE, RW, INC, MIN, MAX, IdentityMap, IterationIndex, i,
                    IterationSpace, Halo, Access)
-from .codegen import LoopSource, ChainSource
+from .codegen import LoopSource
 from .exceptions import MatTypeError, DatTypeError
 from .logger import debug, warning
 from .mpi import MPI
@@ -124,8 +124,17 @@ class _DeviceData(object):
         ds = getattr(self, '_dataset', None)
         return layout.of(ds) if ds is not None else layout.IDENTITY
 
+    def _soa_device(self):
+        """Stored component-major on the device (structure of arrays):
+        component k of row r at k * rows + r (reference: device.py:153-174).
+        One-component Dats are the same either way."""
+        return bool(getattr(self, '_soa', False)) and self.cdim > 1
+
     def _upload(self):
-        _h2d(self._dev, self._layout().to_device_order(self._data))
+        a = self._layout().to_device_order(self._data)
+        if self._soa_device():
+            a = np.ascontiguousarray(a.reshape(len(a), -1).T)
+        _h2d(self._dev, a)
 
     def _to_device(self):
         if self._dev is None:
@@ -140,11 +149,18 @@ class _DeviceData(object):
     def _sync_to_host(self):
         if getattr(self, '_state', HOST) == DEVICE:
             lay = self._layout()
-            if lay.identity:
+            soa = self._soa_device()
+            if lay.identity and not soa:
                 _d2h(self._data, self._dev)
             else:
-                tmp = np.empty_like(self._data)
-                _d2h(tmp, self._dev)
+                if soa:
+                    n = len(self._data)
+                    t = np.empty((self.cdim, n), dtype=self._data.dtype)
+                    _d2h(t, self._dev)
+                    tmp = np.ascontiguousarray(t.T).reshape(self._data.shape)
+                else:
+                    tmp = np.empty_like(self._data)
+                    _d2h(tmp, self._dev)
                 w = self._data.flags.writeable
                 if not w and self._data.flags['OWNDATA']:
                     self._data.setflags(write=True)
@@ -418,6 +434,10 @@ class Solver(base.Solver):
                                       "and 'none'" % p['preconditioner'])
         if p['plot_convergence']:
             p['monitor_convergence'] = True
+        for d in (x, b):
+            if d._soa_device():
+                raise NotImplementedError("solve on a structure-of-arrays Dat: the matrix rows "
+                                          "interleave its components")
         if method:
             return self._krylov(method, A, x, b)
         A._check_err()
@@ -1249,7 +1269,7 @@ class ParLoop(base.ParLoop):
         written = set(id(a.data) for a in args if not a._is_global and a.access is not READ)
         out = []
         for k, a in enumerate(args):
-            if (a._is_indirect and a.access is READ and not a._is_vec_map and
+            if (a._is_indirect and a.access is READ and not a._is_vec_map and not a._is_soa and
                     not isinstance(a.idx, IterationIndex) and a.data.dtype == np.float64 and
                     id(a.data) not in written and a.data.cdim <= 16 and
                     layout._group(a.data.dataset) is tgroup):
@@ -1269,6 +1289,9 @@ class ParLoop(base.ParLoop):
         """(cache key, generated source) of this loop's signature for
         ``mode`` under the current configuration."""
         key = base.JITModule._cache_key(self.kernel, self._it_space.extents, *self.args)
+        # structure-of-arrays Dats are indexed with their row count compiled in
+        key += (tuple(a.data.dataset.total_size if a._is_dat and a._is_soa else 0
+                      for a in self.args),)
         pf = int(cfg.get('hip_owner_prefetch', 1))
         sr = bool(cfg.get('hip_stage_reads', False))
         sw = bool(cfg.get('hip_stage_writes', True))
@@ -1351,319 +1374,8 @@ class ParLoop(base.ParLoop):
         return ent
 
     def compute(self):
-        if _deferred is not None:
-            # inside a fusing capture: run when the captured sequence ends
-            _deferred.append(self)
-            return
         self._entry_cached().run()
         self.maybe_set_halo_update_needed()
-
-
-# -- chains of owner loops (fused inside captured graphs) ---------------------
-
-def owner_plan_for(iterset, n, args, k):
-    """Owner plan of a loop with a given owned range ``k`` (cached)."""
-    cols = _owner_columns(args)
-    key = (id(iterset), n, k, 'fixed', tuple((id(m), i) for m, i in cols))
-    p = _owner_cache.get(key)
-    if p is None:
-        lay = layout.of(iterset)
-        order = None if lay.identity else lay.iperm[:n]
-        vals = [(m._layout_values()[:n], m.dim, i) for m, i in cols]
-        p = OwnerPlan(n, cols[0][0].dataset.total_size, vals, k, order)
-        p._keep = [m for m, _ in cols]
-        _owner_cache[key] = p
-    return p
-
-
-def _xcd_front_order(front):
-    """Block order moving the blocks flagged in ``front`` to the start of
-    the run of blocks each XCD executes (the owner kernels give XCD x the
-    x-th of 8 contiguous runs), keeping the order otherwise."""
-    nb = len(front)
-    q, rr = nb // 8, nb % 8
-    order = []
-    start = 0
-    for x in range(8):
-        n = q + (1 if x < rr else 0)
-        idx = np.arange(start, start + n)
-        order.append(np.concatenate([idx[front[idx]], idx[~front[idx]]]))
-        start += n
-    return np.concatenate(order).astype(np.int64)
-
-
-class ChainPlan(object):
-    """Owner plans of consecutive loops over one partition of the targets
-    (same owned range ``k``) merged into one: a block's slots hold loop 0's
-    elements, then loop 1's, ...; each target's contributions are loop 0's
-    (in its element order), then loop 1's, ..., as codes slot * nac + column
-    with ``nac`` the widest loop's."""
-
-    def __init__(self, plans, nacs):
-        k, ntargets = plans[0].k, plans[0].ntargets
-        self.k, self.ntargets = k, ntargets
-        self.nac = max(nacs)
-        js = [(p.blk_t0 // k).astype(np.int64) for p in plans]
-        J = np.unique(np.concatenate(js)) if js else np.zeros(0, dtype=np.int64)
-        nb = len(J)
-        self.nactive = nb
-        self.blk_t0 = (J * k).astype(np.int32)
-        self.blk_t1 = np.minimum(self.blk_t0.astype(np.int64) + k, ntargets).astype(np.int32)
-        nex = np.zeros((len(plans), nb), dtype=np.int64)
-        self.parts = []
-        for i, p in enumerate(plans):
-            pos = np.searchsorted(J, js[i])
-            rlo = np.zeros(nb, dtype=np.int32)
-            rn = np.zeros(nb, dtype=np.int32)
-            cnt = np.zeros(nb, dtype=np.int64)
-            rlo[pos] = p.rng_lo
-            rn[pos] = p.rng_n
-            cnt[pos] = np.diff(p.ex_off)
-            ex_off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
-            nex[i] = rn + cnt
-            self.parts.append({'rng_lo': rlo, 'rng_n': rn, 'ex_off': ex_off,
-                               'exec': np.ascontiguousarray(p.exec, dtype=np.int32)})
-        sb = np.concatenate([np.zeros((1, nb), dtype=np.int64), np.cumsum(nex, axis=0)[:-1]])
-        self.max_exec = int(nex.sum(axis=0).max()) if nb else 0
-        # contributions, target by target, loop by loop
-        tg, key2, codes = [], [], []
-        for i, p in enumerate(plans):
-            cnt = np.diff(p.tp)
-            t = np.repeat(np.arange(ntargets, dtype=np.int64), cnt)
-            c = p.contrib.astype(np.int64)
-            slot, ac = c // nacs[i], c % nacs[i]
-            b = np.searchsorted(J, t // k)
-            codes.append((slot + sb[i][b]) * self.nac + ac)
-            tg.append(t)
-            key2.append(np.full(len(t), i, dtype=np.int64))
-        t = np.concatenate(tg)
-        order = np.lexsort((np.arange(len(t)), np.concatenate(key2), t))
-        self.contrib = np.concatenate(codes)[order].astype(np.int32)
-        counts = np.bincount(t, minlength=ntargets)
-        self.tp = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
-        touched = np.nonzero(counts)[0].astype(np.int32)
-        self.tgt = touched
-        self.tg_off = np.searchsorted(touched, np.concatenate([self.blk_t0, self.blk_t1[-1:]])
-                                      ).astype(np.int32) if nb else np.zeros(1, dtype=np.int32)
-        if nb:
-            self.tg_off[-1] = np.searchsorted(touched, self.blk_t1[-1])
-        self.tg_cb = np.append(self.tp[touched], self.tp[-1]).astype(np.int32)
-        self.c16 = self.max_exec * self.nac <= 0xffff
-        # blocks that run elements of later loops take longer: start them
-        # first within their XCD's run of blocks, so they do not finish last
-        heavy = (nex[1:] > 0).any(axis=0) if len(plans) > 1 else np.zeros(nb, dtype=bool)
-        self._permute(_xcd_front_order(heavy))
-        self._dev = None
-
-    def _permute(self, order):
-        """Reorder the blocks (per-block arrays and the per-block segments
-        of the element and touched-target lists)."""
-        def segs(off, arr):
-            parts = [arr[off[b]:off[b + 1]] for b in order]
-            new = np.concatenate(parts) if parts else arr[:0]
-            noff = np.concatenate([[0], np.cumsum([len(x) for x in parts])]).astype(np.int32)
-            return noff, new
-        self.blk_t0, self.blk_t1 = self.blk_t0[order], self.blk_t1[order]
-        for part in self.parts:
-            part['rng_lo'], part['rng_n'] = part['rng_lo'][order], part['rng_n'][order]
-            part['ex_off'], part['exec'] = segs(part['ex_off'], part['exec'])
-        g = self.tg_off
-        tcb = self.tg_cb[:-1]
-        parts_t = [self.tgt[g[b]:g[b + 1]] for b in order]
-        parts_c = [tcb[g[b]:g[b + 1]] for b in order]
-        self.tgt = np.concatenate(parts_t).astype(np.int32) if parts_t else self.tgt
-        self.tg_cb = np.append(np.concatenate(parts_c) if parts_c else tcb,
-                               self.tg_cb[-1]).astype(np.int32)
-        self.tg_off = np.concatenate([[0], np.cumsum([len(x) for x in parts_t])]).astype(np.int32)
-
-    def device_arrays(self):
-        if self._dev is None:
-            d = {}
-
-            def up(name, a):
-                a = np.ascontiguousarray(a)
-                t = _dev_bytes(a.nbytes)
-                _h2d(t, a)
-                d[name] = t
-            up('blk_t0', self.blk_t0)
-            up('blk_t1', self.blk_t1)
-            for j, part in enumerate(self.parts):
-                for name in ('rng_lo', 'rng_n', 'ex_off', 'exec'):
-                    up('%s%d' % (name, j), part[name])
-            up('tp', self.tp)
-            up('contrib', self.contrib.astype(np.uint16) if self.c16 else self.contrib)
-            up('tg_off', self.tg_off)
-            up('tgt', np.append(self.tgt, 0).astype(np.int32))
-            up('tg_cb', np.append(self.tg_cb, self.tg_cb[-1]).astype(np.int32))
-            self._dev = d
-        return self._dev
-
-
-def _chain_target(pl):
-    """The Dat an owner loop increments, if the loop may join a chain."""
-    if pl._it_space.extents:
-        return None
-    if any(a._is_mat or a._is_global_reduction for a in pl.args):
-        return None
-    if pl._mode() != 'owner':
-        return None
-    race = [a for a in pl.args if a._is_indirect and a.access is INC]
-    return race[0].data if race else None
-
-
-class _ChainEntry(object):
-    """One launch running a chain of owner loops into the same Dat."""
-
-    def __init__(self, loops, bs):
-        consts = base.Const._definitions()
-        self.consts = consts
-        pf = int(cfg.get('hip_owner_prefetch', 1))
-        waves = int(cfg.get('hip_chain_waves', 8))
-        key = tuple(base.JITModule._cache_key(pl.kernel, pl._it_space.extents, *pl.args)
-                    for pl in loops) + ('chain', bs, pf, waves)
-        loop = _loop_cache.get(key)
-        if loop is None:
-            src = ChainSource([(pl.kernel, pl.args) for pl in loops], consts, bs, prefetch=pf)
-            src.min_waves = waves
-            loop = _CompiledLoop(src)
-            _loop_cache[key] = loop
-        self.loop = loop
-        src = loop.source
-        self.to_device = []
-        for pl in loops:
-            iterset = pl._it_space.iterset
-            for a in pl.args:
-                if a._is_direct and a.data.dataset is not iterset:
-                    layout.link(a.data.dataset, iterset)
-            m0, i0 = _owner_columns(pl.args)[0]
-            layout.prefer(iterset, m0, max(i0, 0))
-            layout.of(iterset)
-            for a in pl.args:
-                if (a._is_dat or a._is_global) and a.data not in self.to_device:
-                    self.to_device.append(a.data)
-        for d in self.to_device:
-            d._to_device()
-        # the first loop sizes the owned range; the others follow it
-        pl0 = loops[0]
-        n0 = pl0._it_space.exec_size if pl0.needs_exec_halo else pl0._it_space.size
-        p0 = get_owner_plan(pl0._it_space.iterset, n0, pl0.args, src.owner_slot_bytes(), block=bs)
-        if p0 is None:
-            raise _NoOwnerPlan()
-        plans = [p0]
-        for pl in loops[1:]:
-            n = pl._it_space.exec_size if pl.needs_exec_halo else pl._it_space.size
-            plans.append(owner_plan_for(pl._it_space.iterset, n, pl.args, p0.k))
-        nacs = [sum(c for _, _, c in src.part_columns(j)) for j in range(len(loops))]
-        plan = ChainPlan(plans, nacs)
-        limit = cfg.get('hip_owner_lds') or min(OWNER_LDS_LIMIT, _state.get('lds_per_block') or 65536)
-        lds = src.owner_lds_bytes(plan.k, plan.max_exec, 0)
-        if lds > limit:
-            raise _NoOwnerPlan()
-        pdev = plan.device_arrays()
-        args = [a for pl in loops for a in pl.args]
-        vals = []
-        for role, pos in src.param_roles():
-            a = args[pos] if pos is not None else None
-            if role == 'start':
-                vals.append(0)
-            elif role == 'count':
-                vals.append(plan.nactive)
-            elif role == 'own_pitch':
-                vals.append(plan.max_exec)
-            elif role == 'own_c16':
-                vals.append(int(plan.c16))
-            elif role.startswith('own_'):
-                name = {'own_t0': 'blk_t0', 'own_t1': 'blk_t1', 'own_tgoff': 'tg_off',
-                        'own_tgt': 'tgt', 'own_tgcb': 'tg_cb', 'own_tp': 'tp',
-                        'own_contrib': 'contrib'}.get(role)
-                if name is None:
-                    for base_name, field in (('own_rlo', 'rng_lo'), ('own_rn', 'rng_n'),
-                                             ('own_exoff', 'ex_off'), ('own_exec', 'exec')):
-                        if role.startswith(base_name) and role[len(base_name):].isdigit():
-                            name = field + role[len(base_name):]
-                vals.append(_ptr(pdev[name]))
-            elif role in ('gbl', 'data'):
-                vals.append(_ptr(a.data._dev))
-            elif role == 'map':
-                vals.append(_ptr(a.map._device_values()))
-            else:
-                raise RuntimeError("unknown parameter role %s" % role)
-        self.keep = [loops, plans, plan]
-        self.launches = [(loop.fn, plan.nactive, bs, _argv(vals), lds)] if plan.nactive else []
-        self.written = [loops[0].args[[a._is_indirect and a.access is INC
-                                       for a in loops[0].args].index(True)].data]
-        self.dats = [a.data for a in args if a._is_dat]
-
-    def prepare(self):
-        """Upload changed Consts and host-side data (synchronising work that
-        cannot run inside a graph capture)."""
-        s = stream()
-        loop = self.loop
-        for c in self.consts:
-            if loop.const_versions.get(c.name) != (id(c), c._version):
-                ptr, nbytes = loop.module.global_symbol(c.name)
-                data = np.ascontiguousarray(c.data)
-                nat.check(nat.lib.op2hip_set_global(ptr, data.ctypes.data_as(ctypes.c_void_p),
-                                                    min(nbytes, data.nbytes), ctypes.c_void_p(s)),
-                          'const upload')
-                loop.const_versions[c.name] = (id(c), c._version)
-        for d in self.to_device:
-            if d._state == HOST:
-                d._to_device()
-
-    def run(self):
-        self.prepare()
-        sp = ctypes.c_void_p(stream())
-        for fn, grid, bs, (arr, n), lds in self.launches:
-            nat.check(nat.lib.op2hip_launch(fn, grid, bs, lds, sp, arr, n), 'kernel launch')
-        for d in self.written:
-            d._state = DEVICE
-        for d in self.dats:
-            maybe_setflags(d._data, write=False)
-
-
-_chain_cache = {}
-
-
-def _chain_entry(loops):
-    """Cached chain entry of ``loops`` (None when they cannot chain)."""
-    bs = cfg.get('hip_owner_block_size') or _block_size()
-    key = tuple(pl._call_key() for pl in loops) + (bs,)
-    if key in _chain_cache:
-        return _chain_cache[key]
-    try:
-        ent = _ChainEntry(loops, bs)
-    except _NoOwnerPlan:
-        ent = None
-    _chain_cache[key] = ent
-    return ent
-
-
-def run_chained(loops, launch=True):
-    """Run deferred ``loops`` in order, fusing runs of consecutive owner
-    loops that increment the same Dat into one launch; with ``launch``
-    False only prepare (compile, plan, upload) what running would need."""
-    i = 0
-    while i < len(loops):
-        grp = [loops[i]]
-        tgt = _chain_target(loops[i]) if cfg.get('hip_chain', True) else None
-        if tgt is not None:
-            while i + len(grp) < len(loops) and _chain_target(loops[i + len(grp)]) is tgt:
-                grp.append(loops[i + len(grp)])
-        ent = _chain_entry(grp) if len(grp) > 1 else None
-        if ent is None:
-            for pl in grp:
-                ent1 = pl._entry_cached()
-                if launch:
-                    ent1.run()
-                    pl.maybe_set_halo_update_needed()
-                else:
-                    ent1.prepare()
-        elif launch:
-            ent.run()
-        else:
-            ent.prepare()
-        i += len(grp)
 
 
 _key_cache = {'cfg': (None, None), 'const': (None, None)}
@@ -1763,46 +1475,23 @@ def solve(M, x, b):
     Solver().solve(M, x, b)
 
 
-_deferred = None
-
-
 class Graph(object):
     """A captured sequence of op2 calls, replayed as one hipGraph.
 
     ``fn`` must have run eagerly before (so loops are compiled, plans built
     and scratch allocated) and must not touch host data (``.data``) or change
     Consts.  Replaying re-executes every captured launch on the current
-    stream; host-side coherence states are those left by the capture.
+    stream; host-side coherence states are those left by the capture."""
 
-    With ``fuse`` the loops ``fn`` issues are collected first and then
-    launched with runs of consecutive owner-computes loops that increment
-    the same Dat fused into one kernel (their sums keep the sequential
-    order, so results are unchanged); chain kernels and plans are prepared
-    before the capture starts."""
-
-    def __init__(self, fn, fuse=False):
-        global _deferred
-        loops = None
-        if fuse:
-            prev, _deferred = _deferred, []
-            try:
-                fn()
-                loops = _deferred
-            finally:
-                _deferred = prev
-            run_chained(loops, launch=False)
+    def __init__(self, fn):
         self._g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self._g):
-            if loops is not None:
-                run_chained(loops)
-            else:
-                fn()
+            fn()
 
     def replay(self):
         self._g.replay()
 
 
-def capture(fn, fuse=False):
-    """Capture ``fn`` (a sequence of par_loops) into a :class:`Graph`;
-    ``fuse`` chains consecutive owner loops into one Dat (see Graph)."""
-    return Graph(fn, fuse)
+def capture(fn):
+    """Capture ``fn`` (a sequence of par_loops) into a :class:`Graph`."""
+    return Graph(fn)

@@ -208,7 +208,9 @@ BRES_CALC_VEC = _vector_form(
 class Airfoil(object):
     """Sets, maps, dats and kernels of one airfoil problem in world ``W``."""
 
-    def __init__(self, W, mesh, q0=None, consts=True):
+    def __init__(self, W, mesh, q0=None, consts=True, soa=False):
+        """``soa`` stores q, q_old and res (four components per cell) as
+        structure-of-arrays Dats."""
         self.W = W
         self.mesh = mesh
         nodes = W.Set(mesh.nnode, 2, 'nodes')
@@ -228,10 +230,10 @@ class Airfoil(object):
         q0 = initial_q(mesh.ncell) if q0 is None else q0
         self.p_bound = W.Dat(bedges, mesh.bound.copy(), np.int32, 'p_bound')
         self.p_x = W.Dat(nodes, mesh.x.copy(), np.float64, 'p_x')
-        self.p_q = W.Dat(vcells, q0.copy(), np.float64, 'p_q')
-        self.p_qold = W.Dat(vcells, np.zeros((mesh.ncell, 4)), np.float64, 'p_qold')
+        self.p_q = W.Dat(vcells, q0.copy(), np.float64, 'p_q', soa=soa)
+        self.p_qold = W.Dat(vcells, np.zeros((mesh.ncell, 4)), np.float64, 'p_qold', soa=soa)
         self.p_adt = W.Dat(cells, np.zeros(mesh.ncell), np.float64, 'p_adt')
-        self.p_res = W.Dat(vcells, np.zeros((mesh.ncell, 4)), np.float64, 'p_res')
+        self.p_res = W.Dat(vcells, np.zeros((mesh.ncell, 4)), np.float64, 'p_res', soa=soa)
         self.rms = W.Global(1, 0.0, np.float64, 'rms')
         self.consts = []
         if consts:

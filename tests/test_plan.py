@@ -247,38 +247,3 @@ def test_mat_scatter_program(rd, cd, k):
                     seen.add((e, i0, j))
                 assert got == sorted(got)
     assert len(seen) == n * 9
-
-
-def test_chain_plan_merges_loops_in_order():
-    """Two owner loops into the same cells (interior edges, then boundary
-    edges) merged on one partition: every block's slots hold loop 0's
-    elements then loop 1's; each target's contributions are loop 0's in its
-    element order, then loop 1's; each (loop, element, column) once."""
-    from pyop2_amd.hip import ChainPlan
-    m = meshes.QuadMesh(40, 20, seed=7)
-    pa = OwnerPlan(m.nedge, m.ncell, [(m.pecell, 2, 0), (m.pecell, 2, 1)], 32)
-    pb = OwnerPlan(m.nbedge, m.ncell, [(m.pbecell, 1, 0)], 32)
-    ch = ChainPlan([pa, pb], [2, 1])
-    assert ch.nac == 2
-    cols = [[m.pecell[:, 0], m.pecell[:, 1]], [m.pbecell[:, 0]]]
-    seen = set()
-    for b in range(ch.nactive):
-        els = []
-        for part in ch.parts:
-            ex = part['exec'][part['ex_off'][b]:part['ex_off'][b + 1]].view(np.uint32)
-            rn = int(part['rng_n'][b])
-            els.append(np.concatenate([np.arange(part['rng_lo'][b], part['rng_lo'][b] + rn),
-                                       (ex & 0x7fffffff).astype(np.int64)]))
-        bounds = np.cumsum([0] + [len(e) for e in els])
-        assert bounds[-1] <= ch.max_exec
-        for t in range(ch.blk_t0[b], ch.blk_t1[b]):
-            prev = (-1, -1)
-            for code in ch.contrib[ch.tp[t]:ch.tp[t + 1]]:
-                slot, ac = divmod(int(code), ch.nac)
-                i = int(np.searchsorted(bounds, slot, side='right') - 1)
-                e = int(els[i][slot - bounds[i]])
-                assert cols[i][ac][e] == t
-                assert (i, e) >= prev
-                prev = (i, e)
-                seen.add((i, e, ac))
-    assert len(seen) == 2 * m.nedge + m.nbedge
