@@ -164,6 +164,23 @@ int op2hip_mat_program(int nactive, const int* blk_t0, const int* blk_t1, const 
                        const int* cmap, int carity, int e1, int nnodes, const int* rowptr,
                        const int* colidx, int rd, int cd, int wide, int* poff, void* prog);
 
+/* The same program built on the device from the device owner plan's
+ * arrays (contrib uint16 when c16, else int32) and the device CSR: poff
+ * (device, npairs + 1) and prog (device, nelems * nac * e1 codes, uint16
+ * unless wide).  One thread per row node walks its contributions in plan
+ * order, so each node pair receives its entries in sequential order
+ * without atomics.  work: op2hip_mat_program_workspace(npairs) bytes.
+ * Waits for the stream once (to report an element naming an entry outside
+ * the pattern). */
+long long op2hip_mat_program_workspace(long long npairs);
+int op2hip_mat_program_device(int nactive, const int* blk_t0, const int* blk_t1,
+                              const int* ex_off, const int* exec, const int* rng_lo,
+                              const int* rng_n, const int* tp, const void* contrib, int c16,
+                              int nac, const int* cmap, int carity, int e1, int nnodes,
+                              const int* rowptr, const int* colidx, int rd, int cd,
+                              long long npairs, int wide, void* work, long long work_bytes,
+                              int* poff, void* prog, void* stream);
+
 /* ---- data layout ----------------------------------------------------------------
  * No counterpart in the reference, which stores every Set in user order.
  * Orderings that renumber a Set's device storage for locality (user-visible

@@ -94,6 +94,12 @@ _SIGS = {
                                          c_void]),
     'op2hip_owner_plan_dfill': (c_int, [c_void] + [c_void] * 11 + [c_int, c_void]),
     'op2hip_owner_plan_ddiscard': (c_int, [c_void]),
+    'op2hip_mat_program_workspace': (c_ll, [c_ll]),
+    'op2hip_mat_program_device': (c_int, [c_int] + [c_void] * 8 + [c_int, c_int, c_void, c_int,
+                                                                  c_int, c_int, c_void, c_void,
+                                                                  c_int, c_int, c_ll, c_int,
+                                                                  c_void, c_ll, c_void, c_void,
+                                                                  c_void]),
     'op2hip_mat_program': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void, c_void,
                                    c_void, c_int,
                                    c_void, c_int, c_int, c_int, c_void, c_void, c_int, c_int,

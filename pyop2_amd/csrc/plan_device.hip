@@ -680,3 +680,159 @@ int op2hip_owner_plan_ddiscard(void* handle) {
 }
 
 }  // extern "C"
+
+// ---- owner Mat assembly program on the device -----------------------------
+
+namespace {
+
+struct MatPlan {
+  const int *blk_t0, *blk_t1, *ex_off, *exec, *rng_lo, *rng_n, *tp;
+  const void* contrib;
+  int c16, nac, nactive;
+};
+
+__device__ inline int contrib_at(const MatPlan& P, int c) {
+  return P.c16 ? static_cast<int>(static_cast<const unsigned short*>(P.contrib)[c])
+               : static_cast<const int*>(P.contrib)[c];
+}
+
+// Pair of (row node n, column node m): the position of m among the node
+// columns of scalar row n*rd (every cd-th entry), as a global pair index;
+// -1 when the pattern lacks it.
+__device__ inline long long pair_of(const int* __restrict__ rowptr,
+                                    const int* __restrict__ colidx, int rd, int cd, int n, int m) {
+  const int s0 = rowptr[static_cast<long long>(n) * rd];
+  const int s1 = rowptr[static_cast<long long>(n) * rd + 1];
+  int lo = 0, hi = (s1 - s0) / cd - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int v = colidx[s0 + mid * cd] / cd;
+    if (v == m) return static_cast<long long>(s0) / (static_cast<long long>(rd) * cd) + mid;
+    if (v < m)
+      lo = mid + 1;
+    else
+      hi = mid - 1;
+  }
+  return -1;
+}
+
+// One thread per row node: its contributions in plan order (ascending user
+// element, then row position) and each one's column positions j, so the
+// pairs of node n, touched by this thread alone, receive their entries in
+// sequential order without atomics.  COUNT: entries per pair; else the
+// codes, through per-pair cursors (the scanned offsets).
+template <bool COUNT>
+__global__ void mat_program_rows(MatPlan P, const int* __restrict__ cmap, int carity, int e1,
+                                 int nnodes, const int* __restrict__ rowptr,
+                                 const int* __restrict__ colidx, int rd, int cd,
+                                 int* __restrict__ cnt, int wide, void* __restrict__ prog,
+                                 int* __restrict__ err) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  const int c0 = P.tp[n], c1 = P.tp[n + 1];
+  if (c0 == c1) return;
+  // the active block owning n: the last whose first target is <= n
+  int lo = 0, hi = P.nactive - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (P.blk_t0[mid] <= n)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int b = lo;
+  const int rn = P.rng_n[b], rlo = P.rng_lo[b], ex0 = P.ex_off[b];
+  const int cbase = P.tp[P.blk_t0[b]];
+  for (int c = c0; c < c1; ++c) {
+    const int slot = contrib_at(P, c) / P.nac;
+    const int e = slot < rn ? rlo + slot
+                            : static_cast<int>(static_cast<unsigned>(P.exec[ex0 + slot - rn]) &
+                                               0x7fffffffu);
+    const long long code0 = static_cast<long long>(c - cbase) * e1;
+    for (int j = 0; j < e1; ++j) {
+      const long long p =
+          pair_of(rowptr, colidx, rd, cd, n, cmap[static_cast<long long>(e) * carity + j]);
+      if (p < 0) {
+        atomicOr(err, 1);
+        continue;
+      }
+      if (COUNT) {
+        ++cnt[p];
+      } else {
+        const int at = cnt[p]++;
+        const long long code = code0 + j;
+        if (wide)
+          static_cast<int*>(prog)[at] = static_cast<int>(code);
+        else if (code > 0xffff)
+          atomicOr(err, 2);
+        else
+          static_cast<unsigned short*>(prog)[at] = static_cast<unsigned short>(code);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+long long op2hip_mat_program_workspace(long long npairs) {
+  if (npairs < 0) return -1;
+  return static_cast<long long>(align256(sizeof(int) * (npairs + 1)) +
+                                align256(sizeof(long long) * scan_tiles_for(npairs + 1)) +
+                                align256(sizeof(long long) * 2) + align256(sizeof(int)));
+}
+
+// The scatter program of op2hip_mat_program built on the device from the
+// device owner plan (contrib uint16 when c16, else int32) and the device
+// CSR: poff (device, npairs + 1) and prog (device, tp[nnodes] * e1 codes,
+// uint16 unless wide).  work: op2hip_mat_program_workspace(npairs) bytes.
+// Waits for the stream once, to report an element naming an entry outside
+// the pattern or a code that does not fit 16 bits.
+int op2hip_mat_program_device(int nactive, const int* blk_t0, const int* blk_t1,
+                              const int* ex_off, const int* exec, const int* rng_lo,
+                              const int* rng_n, const int* tp, const void* contrib, int c16,
+                              int nac, const int* cmap, int carity, int e1, int nnodes,
+                              const int* rowptr, const int* colidx, int rd, int cd,
+                              long long npairs, int wide, void* work, long long work_bytes,
+                              int* poff, void* prog, void* stream) {
+  OP2HIP_REQUIRE(nactive >= 0 && nac > 0 && carity > 0 && e1 > 0 && e1 <= carity && rd > 0 &&
+                     cd > 0 && npairs >= 0 && poff && prog && work,
+                 "bad Mat program arguments");
+  OP2HIP_REQUIRE(npairs <= 0x7fffffffLL, "Mat pattern of %lld node pairs", npairs);
+  OP2HIP_REQUIRE(work_bytes >= op2hip_mat_program_workspace(npairs), "Mat program workspace");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  char* base = static_cast<char*>(work);
+  int* cnt = reinterpret_cast<int*>(base);
+  size_t off = align256(sizeof(int) * (npairs + 1));
+  long long* sums = reinterpret_cast<long long*>(base + off);
+  off += align256(sizeof(long long) * scan_tiles_for(npairs + 1));
+  long long* total = reinterpret_cast<long long*>(base + off);
+  off += align256(sizeof(long long) * 2);
+  int* err = reinterpret_cast<int*>(base + off);
+  MatPlan P{blk_t0, blk_t1, ex_off, exec, rng_lo, rng_n, tp, contrib, c16, nac, nactive};
+  hipLaunchKernelGGL(zero_ints, dim3((npairs + 1 + 255) / 256), dim3(256), 0, s, cnt,
+                     npairs + 1);
+  hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, err, 1LL);
+  const dim3 grid((nnodes + 255) / 256 > 0 ? (nnodes + 255) / 256 : 1);
+  if (nactive > 0)
+    hipLaunchKernelGGL(mat_program_rows<true>, grid, dim3(256), 0, s, P, cmap, carity, e1,
+                       nnodes, rowptr, colidx, rd, cd, cnt, wide, prog, err);
+  exclusive_scan(cnt, static_cast<int>(npairs), poff, cnt, sums, total, s);
+  if (nactive > 0)
+    hipLaunchKernelGGL(mat_program_rows<false>, grid, dim3(256), 0, s, P, cmap, carity, e1,
+                       nnodes, rowptr, colidx, rd, cd, cnt, wide, prog, err);
+  int herr = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    op2hip::set_error("Mat program build failed: %s", hipGetErrorString(e));
+    return OP2HIP_ERR_HIP;
+  }
+  OP2HIP_REQUIRE(!(herr & 1), "an element names a matrix entry outside the sparsity pattern");
+  OP2HIP_REQUIRE(!(herr & 2), "Mat program codes do not fit 16 bits");
+  return OP2HIP_OK;
+}
+
+}  // extern "C"

@@ -878,16 +878,41 @@ def _mat_program(plan, arg, e1):
     if hit is not None and hit[0] is sp:
         return hit[1]
     rd, cd = arg.data.dims
-    poff, prog, wide = mat_program_host(
-        plan, arg.map[1]._layout_values(), e1, arg.map[0].dataset.total_size,
-        sp._dev_rowptr.cpu().numpy().view(np.int32), sp._dev_colidx.cpu().numpy().view(np.int32),
-        rd, cd)
-    d_poff, d_prog = _dev_bytes(poff.nbytes), _dev_bytes(prog.nbytes)
-    _h2d(d_poff, poff)
-    _h2d(d_prog, prog)
-    res = (d_poff, d_prog, wide)
+    if isinstance(plan, DeviceOwnerPlan):
+        res = _mat_program_device(plan, arg, e1, sp, rd, cd)
+    else:
+        poff, prog, wide = mat_program_host(
+            plan, arg.map[1]._layout_values(), e1, arg.map[0].dataset.total_size,
+            sp._dev_rowptr.cpu().numpy().view(np.int32),
+            sp._dev_colidx.cpu().numpy().view(np.int32), rd, cd)
+        d_poff, d_prog = _dev_bytes(poff.nbytes), _dev_bytes(prog.nbytes)
+        _h2d(d_poff, poff)
+        _h2d(d_prog, prog)
+        res = (d_poff, d_prog, wide)
     progs[id(sp)] = (sp, res)
     return res
+
+
+def _mat_program_device(plan, arg, e1, sp, rd, cd):
+    """The scatter program built on the device from a device owner plan
+    (csrc/plan_device.hip op2hip_mat_program_device): (poff, prog, wide)."""
+    nnodes = arg.map[0].dataset.total_size
+    npairs = sp._d_nz // (rd * cd)
+    wide = int(max(plan.max_codes, 1) * e1 > 0xffff)
+    ncodes = plan.n * plan.nac * e1
+    d_poff = torch.empty(npairs + 1, dtype=torch.int32, device=device())
+    d_prog = _dev_bytes(max(ncodes, 1) * (4 if wide else 2))
+    work = _dev_bytes(nat.lib.op2hip_mat_program_workspace(npairs))
+    d = plan.device_arrays()
+    P = lambda t: ctypes.c_void_p(_ptr(t))
+    cmap = arg.map[1]
+    nat.check(nat.lib.op2hip_mat_program_device(
+        plan.nactive, P(d['blk_t0']), P(d['blk_t1']), P(d['ex_off']), P(d['exec']),
+        P(d['rng_lo']), P(d['rng_n']), P(d['tp']), P(d['contrib']), int(plan.c16), plan.nac,
+        P(cmap._device_values()), cmap.dim, e1, nnodes, P(sp._dev_rowptr), P(sp._dev_colidx),
+        rd, cd, npairs, wide, P(work), int(work.numel()), P(d_poff), P(d_prog),
+        ctypes.c_void_p(stream())), 'device Mat program')
+    return (d_poff, d_prog, wide)
 
 
 def _owner_columns(args):

@@ -96,3 +96,35 @@ def test_empty_targets_and_elements(hip):
     v = np.array([[3, 3], [3, 9], [9, 3]], dtype=np.int32)
     same(*build_both(hip, [v], 40, [(v, -1)], 4))
     same(*build_both(hip, [v], 40, [(v, 1)], 64))
+
+
+@pytest.mark.parametrize('dim,k', [(1, 9), (2, 5), (1, 1)])
+def test_mat_program_device_matches_host(hip, dim, k):
+    """The owner Mat scatter program built on the device equals the host
+    build (poff and every code) for scalar and 2x2-block patterns."""
+    from pyop2_amd import hip as H, layout
+    mesh = meshes.TetMesh(6, seed=2)
+    en = mesh.elem_node.astype(np.int32)
+    n, nv = en.shape[0], mesh.nvert
+    el = hip.Set(n, 1)
+    nodes = hip.Set(nv, dim)
+    m = hip.Map(el, nodes, 4, en)
+    layout._group(el).layout = layout.IDENTITY
+    layout._group(nodes).layout = layout.IDENTITY
+    sp = hip.Sparsity((m, m))
+    order = np.random.default_rng(k).permutation(n)
+    r = np.empty(n, dtype=np.int32)
+    r[order] = np.arange(n, dtype=np.int32)
+    plan = H.DeviceOwnerPlan(n, nv, [(m, -1)], k, H.torch.from_numpy(r).to(H.device()))
+
+    class _Arg(object):
+        map = (m, m)
+
+    poff_d, prog_d, wide = H._mat_program_device(plan, _Arg, 4, sp, dim, dim)
+    rp = sp._dev_rowptr.cpu().numpy()
+    ci = sp._dev_colidx[:sp.nz].cpu().numpy()
+    poff_h, prog_h, wide_h = H.mat_program_host(plan, en, 4, nv, rp, ci, dim, dim)
+    assert wide == wide_h
+    assert np.array_equal(poff_d.cpu().numpy(), poff_h)
+    pd = prog_d.cpu().numpy().view(np.int32 if wide else np.uint16)[:len(prog_h)]
+    assert np.array_equal(pd, prog_h)
