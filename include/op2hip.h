@@ -283,6 +283,20 @@ int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, 
                double* x, double* work, double rtol, double atol, double dtol, int maxit,
                int* its, double* rnorm, int* reason, void* stream);
 
+/* op2hip_pcg without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results (bit for bit) as op2hip_pcg; the solver's descriptor, chunk list
+ * and partial sums live in a static per-GPU array of about 1.6 MB instead,
+ * so n is at most op2hip_pcg_noalloc_max_rows() (2^24) and larger systems
+ * are refused with an error.  Calls that overlap on one GPU race on that
+ * array.  Waits for the stream. */
+int op2hip_pcg_noalloc(int n, const int* rowptr, const int* colidx, const double* vals,
+                       const double* b, double* x, double* work, double rtol, double atol,
+                       double dtol, int maxit, int* its, double* rnorm, int* reason,
+                       void* stream);
+int op2hip_pcg_noalloc_max_rows(void);
+
 /* GMRES and BiCGStab for one CSR on the device (Solver.solve with
  * linear_solver 'gmres' or 'bcgs'/'bicgstab', pyop2/petsc_base.py:172-220;
  * the cusp backend's methods, pyop2/cuda.py:504-508): PETSc 3.3 KSPGMRES

@@ -111,6 +111,10 @@ _SIGS = {
     'op2hip_pcg': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void, ctypes.c_double,
                            ctypes.c_double, ctypes.c_double, c_int, P(c_int),
                            P(ctypes.c_double), P(c_int), c_void]),
+    'op2hip_pcg_noalloc': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, c_int,
+                                   P(c_int), P(ctypes.c_double), P(c_int), c_void]),
+    'op2hip_pcg_noalloc_max_rows': (c_int, []),
     'op2hip_order_bisect': (c_int, [c_int, c_int, c_void, c_int, c_int, c_void]),
     'op2hip_order_first_touch': (c_int, [c_int, c_int, c_void, c_int, c_void, c_void]),
     'op2hip_sparsity_workspace': (c_ll, [c_int, c_int, P(c_ll), P(c_int)]),

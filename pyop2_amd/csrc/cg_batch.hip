@@ -496,6 +496,128 @@ int op2hip_cgb_history(void* h, int k, double* out, void* stream) {
 
 int op2hip_cgb_sizeof_system(void) { return static_cast<int>(sizeof(Sys)); }
 
+}  // extern "C"
+
+namespace {
+
+// ---- the single solve without device allocation ----------------------------
+//
+// op2hip_pcg_noalloc keeps what op2hip_pcg allocates per call (the system
+// descriptor, the chunk list and the chunk partials) in this file-scope
+// array instead: one copy per GPU, resident until the process ends.  Host
+// code must not pass the array's name to a kernel (in host code it names a
+// host-side stand-in); it uses the device address that
+// hipGetSymbolAddress returns for the current device, kept per device.
+constexpr int kNoAllocMaxRows = 1 << 24;
+constexpr long long kNoAllocChunks = kNoAllocMaxRows / kChunk + 1;
+constexpr size_t kScratchSys = 256;  // one Sys, rounded up to 256 bytes
+constexpr size_t kScratchChunks = ((sizeof(ChunkRef) * kNoAllocChunks + 255) / 256) * 256;
+constexpr size_t kScratchBytes = kScratchSys + kScratchChunks + sizeof(double) * 2 * kNoAllocChunks;
+static_assert(sizeof(Sys) <= kScratchSys, "system descriptor outgrew its scratch slot");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch (looked up once per device).
+int scratch_for_current_device(unsigned char** out) {
+  int dev = 0;
+  OP2HIP_CHECK(hipGetDevice(&dev));
+  OP2HIP_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d supported", dev,
+                 kMaxDevices);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    OP2HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    OP2HIP_REQUIRE(ptr != nullptr, "no device address for the solver scratch");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return OP2HIP_OK;
+}
+
+// The descriptor and chunk list of one system, written by a kernel (no host
+// copy): chunk c of system 0 is (0, c).
+__global__ void k_noalloc_setup(Sys sy, Sys* __restrict__ sys, ChunkRef* __restrict__ chunks,
+                                int nchunks) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c == 0) sys[0] = sy;
+  if (c < nchunks) chunks[c] = ChunkRef{0, c};
+}
+
+}  // namespace
+
+extern "C" {
+
+int op2hip_pcg_noalloc_max_rows(void) { return kNoAllocMaxRows; }
+
+// op2hip_pcg without device allocation (no counterpart in the reference:
+// for callers that cannot allocate).  Same arguments, results bit for bit
+// those of op2hip_pcg; n at most op2hip_pcg_noalloc_max_rows().  The
+// solver's own state lives in a per-GPU static array: calls that overlap on
+// one GPU (other threads, other streams) race.  Waits for the stream.
+int op2hip_pcg_noalloc(int n, const int* rowptr, const int* colidx, const double* vals,
+                       const double* b, double* x, double* work, double rtol, double atol,
+                       double dtol, int maxit, int* its, double* rnorm, int* reason,
+                       void* stream) {
+  OP2HIP_REQUIRE(n >= 0 && rowptr && colidx && vals && b && x && work && its && rnorm && reason,
+                 "null argument to op2hip_pcg_noalloc");
+  OP2HIP_REQUIRE(n <= kNoAllocMaxRows, "op2hip_pcg_noalloc takes at most %d rows, not %d",
+                 kNoAllocMaxRows, n);
+  unsigned char* base = nullptr;
+  int st = scratch_for_current_device(&base);
+  if (st != OP2HIP_OK) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Batch bt;
+  bt.max_sys = 1;
+  bt.max_chunks = kNoAllocChunks;
+  bt.d_sys = reinterpret_cast<Sys*>(base);
+  bt.d_chunks = reinterpret_cast<ChunkRef*>(base + kScratchSys);
+  bt.d_part = reinterpret_cast<double*>(base + kScratchSys + kScratchChunks);
+  bt.d_hist = nullptr;
+  bt.hist_stride = 0;
+  Sys sy{};
+  sy.rowptr = rowptr;
+  sy.colidx = colidx;
+  sy.vals = vals;
+  sy.b = b;
+  sy.x = x;
+  sy.inv_diag = work;
+  sy.r = work + n;
+  sy.z = work + 2 * (size_t)n;
+  sy.p = work + 3 * (size_t)n;
+  sy.w = work + 4 * (size_t)n;
+  sy.row0 = 0;
+  sy.n = n;
+  sy.chunk0 = 0;
+  sy.nchunks = (n + kChunk - 1) / kChunk;
+  sy.rtol = rtol;
+  sy.atol = atol;
+  sy.dtol = dtol;
+  sy.maxit = maxit;
+  sy.state = SYS_NEW;
+  bt.nsys = 1;
+  bt.nchunks = sy.nchunks;
+  hipLaunchKernelGGL(k_noalloc_setup, dim3((sy.nchunks + 255) / 256 + 1), dim3(256), 0, s, sy,
+                     bt.d_sys, bt.d_chunks, sy.nchunks);
+  OP2HIP_CHECK(hipGetLastError());
+  for (int done = 0;;) {
+    const int k = done ? 16 : 4;
+    if (bt.nchunks > 0) {
+      enqueue_step(&bt, k, s);
+      OP2HIP_CHECK(hipGetLastError());
+    }
+    OP2HIP_CHECK(hipMemcpyAsync(&sy, bt.d_sys, sizeof(Sys), hipMemcpyDeviceToHost, s));
+    OP2HIP_CHECK(hipStreamSynchronize(s));
+    done += k;
+    if (sy.state == SYS_DONE || bt.nchunks == 0 || done > maxit + 16) break;
+  }
+  *its = sy.its;
+  *rnorm = sy.dp;
+  *reason = sy.reason;
+  return OP2HIP_OK;
+}
+
 int op2hip_cgb_chunk_rows(void) { return kChunk; }
 
 

@@ -42,10 +42,12 @@ def pytest_generate_tests(metafunc):
 
 def pytest_collection_modifyitems(items):
     def key(item):
+        # modules named test_zz_* run after everything else
+        last = os.path.basename(str(item.fspath)).startswith('test_zz_')
         try:
-            return item.callspec.getparam('backend')
+            return (last, item.callspec.getparam('backend'))
         except (AttributeError, ValueError):
-            return '_nobackend'
+            return (last, '_nobackend')
     items.sort(key=key)
 
 
