@@ -25,10 +25,18 @@ CANDIDATES = {
     'stream': [{'hip_block_size': b} for b in (128, 256, 512)],
     'owner': [{'hip_owner_block_size': b, 'hip_owner_fill': f}
               for b in (256, 512) for f in (0.6, 0.9, 1.2)],
-    'owner_mat': [{'hip_owner_mat_block_size': b, 'hip_owner_mat_fill': f}
-                  for b in (128, 256, 512) for f in (0.4, 0.6, 0.7, 0.9)],
-    'coloured': [{'hip_partition_size': p, 'hip_block_size': b}
-                 for p in (128, 256, 512, 1024) for b in (128, 256) if b <= p],
+    # Mat blocks: workgroup size, elements per thread, and the LDS a block
+    # may use (0: a CU's share at 8 waves per SIMD; 2x that: fewer, larger
+    # owned ranges with less redundant element work)
+    'owner_mat': [{'hip_owner_mat_block_size': b, 'hip_owner_mat_fill': f,
+                   'hip_owner_mat_lds': lds * b // 256}
+                  for b in (128, 256, 512) for f in (0.4, 0.6, 0.7, 0.9)
+                  for lds in (0, 40960)],
+    # coloured plans: partition size, workgroup size and colouring depth
+    # (two levels, blocks then threads as plan.pyx; or one level, elements)
+    'coloured': [{'hip_partition_size': p, 'hip_block_size': b, 'hip_colouring': c}
+                 for p in (128, 256, 512, 1024) for b in (128, 256) if b <= p
+                 for c in ('blocks', 'elements')],
 }
 
 _table = None
