@@ -93,6 +93,9 @@ def parse():
     p.add_argument('--solvers', type=float, default=2e6,
                    help='elements of the 2-D system the CG, GMRES(30) and BiCGStab '
                         'iterations are timed on (0: skip)')
+    p.add_argument('--opt', action='append', default=[], metavar='KEY=VALUE',
+                   help='set a backend configuration key (repeatable), e.g. '
+                        'hip_owner_mat_block_size=512')
     p.add_argument('--copy-calibration', action='store_true',
                    help='only run a 256 MiB device copy three times (the reference reading '
                         'for rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit')
@@ -150,6 +153,13 @@ def main():
         opts['hip_owner_fill'] = args.owner_fill
     if args.owner_targets:
         opts['hip_owner_targets'] = args.owner_targets
+    for kv in args.opt:
+        k, v = kv.split('=', 1)
+        try:
+            v = json.loads(v)
+        except ValueError:
+            pass
+        opts[k] = v
     op2.init(**opts)
     mesh = meshes.QuadMesh(args.nx, args.ny, seed=0)
     A = (airfoil.AirfoilVector if args.vector_maps else airfoil.Airfoil)(op2, mesh,
@@ -228,6 +238,16 @@ def main():
             tot += ms.value
         return tot / 1e3 / nrep
 
+    # the step as a program written for the reference drives it: each
+    # par_loop called from Python (no graph)
+    eager_steps = max(20, args.steps // 5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(eager_steps):
+        A.iteration()
+    torch.cuda.synchronize()
+    eager_s = (time.perf_counter() - t0) / eager_steps
+
     res_avg_s = hot_avg(A.res_calc)
     res_gbs = ub['res_calc'] / res_avg_s / 1e9
     cold = None
@@ -267,6 +287,7 @@ def main():
                    'unique_bytes_per_step': int(step_bytes),
                    'pct_of_hbm_peak': round(100.0 * value / HBM_PEAK_GBS, 2),
                    'graph': bool(args.graph),
+                   'eager_ms_per_step': round(eager_s * 1e3, 4),
                    'vector_maps': bool(args.vector_maps),
                    'soa': bool(args.soa),
                    'autotune': _tuned() if args.autotune else False,
@@ -307,6 +328,9 @@ def main():
         extra['solvers'] = cgbench.run(op2, int(args.solvers), 200)
     if extra:
         out['extra'] = extra
+    if args.autotune:
+        # every leg's loops, tuned as they first ran
+        out['config']['autotune'] = _tuned()
     print(json.dumps(out))
 
 

@@ -204,13 +204,14 @@ int op2hip_order_first_touch(int ntarget, int nsrc, const int* map, int dim,
  * releases the handle.  rowmaps/colmaps: host arrays of device int32 map
  * pointers (at most 8 map pairs); nelems[m]: iteration-set size of pair m.
  * work: a device buffer of op2hip_sparsity_workspace() bytes (about
- * 4 * (4 * nrows + sum of nelems[m] * rdims[m])), kept untouched until the
- * handle is filled or discarded; the build allocates no device memory.
+ * 4 * (4 * nrows + I * (1 + max cdims)), I the sum of nelems[m] * rdims[m]),
+ * kept untouched until the handle is filled or discarded; the build
+ * allocates no device memory.
  * Build waits for the stream once (nnz), fill once (its check that every
  * row recounted to the build's count).  Bit-exact to the reference's
  * sorted-set construction. */
 long long op2hip_sparsity_workspace(int nrows, int nmaps, const long long* nelems,
-                                    const int* rdims);
+                                    const int* rdims, const int* cdims);
 int op2hip_sparsity_build(int rmult, int cmult, int nrows, int nmaps, const int* const* rowmaps,
                           const int* rdims, const int* const* colmaps, const int* cdims,
                           const long long* nelems, void* work, long long work_bytes,

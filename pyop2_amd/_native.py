@@ -117,7 +117,7 @@ _SIGS = {
     'op2hip_pcg_noalloc_max_rows': (c_int, []),
     'op2hip_order_bisect': (c_int, [c_int, c_int, c_void, c_int, c_int, c_void]),
     'op2hip_order_first_touch': (c_int, [c_int, c_int, c_void, c_int, c_void, c_void]),
-    'op2hip_sparsity_workspace': (c_ll, [c_int, c_int, P(c_ll), P(c_int)]),
+    'op2hip_sparsity_workspace': (c_ll, [c_int, c_int, P(c_ll), P(c_int), P(c_int)]),
     'op2hip_sparsity_build': (c_int, [c_int, c_int, c_int, c_int, P(c_void), P(c_int), P(c_void),
                                       P(c_int), P(c_ll), c_void, c_ll, c_void, P(c_void), P(c_ll),
                                       c_void]),

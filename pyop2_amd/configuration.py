@@ -38,8 +38,8 @@ DEFAULTS = {
     'hip_owner_stage': False,     # owner blocks stage their owned rows of gathered Dats in LDS
     'hip_owner_prefetch': 1,      # owner write-back loads issued before the elements run:
                                   # 0 none, 1 the first target's lookups, 2 also its old value
-    'hip_owner_mat_lds': 0,       # LDS bytes an owner Mat block may use (0: a CU's share at
-                                  # full occupancy, 20 KB for 256 threads)
+    'hip_owner_mat_lds': 0,       # LDS bytes an owner Mat block may use (0: twice a CU's
+                                  # share at full occupancy, 40 KB for 256 threads)
     'hip_owner_waves': 0,         # waves per SIMD owner loops are compiled for (0: compiler's choice)
     'hip_device_plan': True,      # owner plans built on the device (False: the host build)
     'hip_layout': True,           # locality renumbering of device storage

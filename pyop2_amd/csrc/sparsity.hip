@@ -25,13 +25,14 @@
 //      synchronisation: the caller needs nnz to allocate colidx), expand to
 //      rmult x cmult blocks: row rmult*n+r holds {cmult*m+c : m in cols(n),
 //      c < cmult} in order;
-//   4. the fill pass repeats step 2 and writes each row's sorted columns
-//      straight into colidx.
+//   4. the fill pass expands each row's sorted distinct columns (kept by step
+//      2 at the row's first candidate slot) into colidx; long rows repeat
+//      step 2.
 //
 // Work is linear in the candidates (times log^2 of a row's length) and the
-// scratch is the caller's workspace (incidence lists and per-row arrays,
-// about 4 * (4 * nrows + incidences) bytes), so no build allocates device
-// memory.
+// scratch is the caller's workspace (incidence lists, a slot per candidate
+// and per-row arrays: about 4 * (4 * nrows + incidences * (1 + widest
+// column map)) bytes), so no build allocates device memory.
 #include <vector>
 
 #include "common.h"
@@ -140,6 +141,8 @@ struct RowOut {
   int* nnz_rows;     // fill pass, may be null
   int* ucnt;         // count pass: distinct values per row
   int* err;          // set when a fill pass finds another count than the build
+  int* uniq;         // each wavefront row's sorted distinct values, at its first
+                     // candidate slot (written by the count pass, read by the fill)
 };
 
 // Write the sorted distinct values of node row n (rank-th value v) or count
@@ -157,7 +160,7 @@ __device__ inline void put_count(const RowOut& o, int n, int u) {
   for (int rr = 0; rr < o.rmult; ++rr) o.nnz_rows[n * o.rmult + rr] = u * o.cmult;
 }
 
-template <int R, bool FILL>
+template <int R>
 __device__ inline void wave_row(const Pairs& P, const int* __restrict__ inc, int ib, int slots,
                                 int n, int lane, const RowOut& o) {
   int v[R];
@@ -180,22 +183,15 @@ __device__ inline void wave_row(const Pairs& P, const int* __restrict__ inc, int
     bal[r] = __ballot(first);
     u += __popcll(bal[r]);
   }
-  if (!FILL) {
-    if (lane == 0) o.ucnt[n] = u;
-    return;
-  }
-  if (o.nptr[n + 1] - o.nptr[n] != u) {
-    if (lane == 0) atomicOr(o.err, 1);
-    return;
-  }
+  if (lane == 0) o.ucnt[n] = u;
+  int* dst = o.uniq + static_cast<long long>(ib) * P.cmax;
   int before = 0;
   const unsigned long long lt = __lanemask_lt();
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    if ((bal[r] >> lane) & 1ull) put_value(o, n, u, before + __popcll(bal[r] & lt), v[r]);
+    if ((bal[r] >> lane) & 1ull) dst[before + __popcll(bal[r] & lt)] = v[r];
     before += __popcll(bal[r]);
   }
-  if (o.nnz_rows && lane == 0) put_count(o, n, u);
 }
 
 // One wavefront per row; rows of more than kWaveSlots candidates are listed
@@ -216,12 +212,24 @@ __global__ void __launch_bounds__(256) rows_wave(const int* __restrict__ iptr,
   }
   const int sl = static_cast<int>(slots);
   const int row = static_cast<int>(n);
+  if (FILL) {
+    // the build's sorted distinct values, expanded into colidx
+    const int u = o.nptr[n + 1] - o.nptr[n];
+    if (u > sl) {
+      if (lane == 0) atomicOr(o.err, 1);
+      return;
+    }
+    const int* src = o.uniq + static_cast<long long>(ib) * P.cmax;
+    for (int q = lane; q < u; q += 64) put_value(o, row, u, q, src[q]);
+    if (o.nnz_rows && lane == 0) put_count(o, row, u);
+    return;
+  }
   if (sl <= 64)
-    wave_row<1, FILL>(P, inc, ib, sl, row, lane, o);
+    wave_row<1>(P, inc, ib, sl, row, lane, o);
   else if (sl <= 128)
-    wave_row<2, FILL>(P, inc, ib, sl, row, lane, o);
+    wave_row<2>(P, inc, ib, sl, row, lane, o);
   else
-    wave_row<4, FILL>(P, inc, ib, sl, row, lane, o);
+    wave_row<4>(P, inc, ib, sl, row, lane, o);
 }
 
 __device__ inline unsigned hash_slot(int v) {
@@ -380,12 +388,12 @@ __global__ void expand_rowptr(const int* __restrict__ nptr, int nrows, int rmult
 
 // Workspace layout (offsets in bytes, each segment 256-byte aligned).
 struct Work {
-  size_t cnt, iptr, inc, ucnt, nptr, longlist, nlong, err, sums, total, bytes;
+  size_t cnt, iptr, inc, uniq, ucnt, nptr, longlist, nlong, err, sums, total, bytes;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
-Work work_layout(long long nrows, long long ninc) {
+Work work_layout(long long nrows, long long ninc, int cmax) {
   Work w;
   size_t o = 0;
   auto take = [&](size_t b) {
@@ -398,6 +406,7 @@ Work work_layout(long long nrows, long long ninc) {
   w.cnt = take(rows1);
   w.iptr = take(rows1);
   w.inc = take(sizeof(int) * static_cast<size_t>(ninc));
+  w.uniq = take(sizeof(int) * static_cast<size_t>(ninc) * cmax);
   w.ucnt = take(rows1);
   w.nptr = take(rows1);
   w.longlist = take(sizeof(int) * static_cast<size_t>(nrows + 1));
@@ -434,11 +443,15 @@ long long incidences(int nmaps, const long long* nelems, const int* rdims) {
 extern "C" {
 
 long long op2hip_sparsity_workspace(int nrows, int nmaps, const long long* nelems,
-                                    const int* rdims) {
-  if (nrows < 0 || nmaps <= 0 || nmaps > kMaxPairs || !nelems || !rdims) return -1;
-  for (int m = 0; m < nmaps; ++m)
-    if (nelems[m] < 0 || rdims[m] <= 0) return -1;
-  return static_cast<long long>(work_layout(nrows, incidences(nmaps, nelems, rdims)).bytes);
+                                    const int* rdims, const int* cdims) {
+  if (nrows < 0 || nmaps <= 0 || nmaps > kMaxPairs || !nelems || !rdims || !cdims) return -1;
+  int cmax = 1;
+  for (int m = 0; m < nmaps; ++m) {
+    if (nelems[m] < 0 || rdims[m] <= 0 || cdims[m] <= 0) return -1;
+    cmax = cdims[m] > cmax ? cdims[m] : cmax;
+  }
+  return static_cast<long long>(
+      work_layout(nrows, incidences(nmaps, nelems, rdims), cmax).bytes);
 }
 
 // Phase 1: build the pattern and write rowptr (device, nrows*rmult + 1).
@@ -465,7 +478,7 @@ int op2hip_sparsity_build(int rmult, int cmult, int nrows, int nmaps, const int*
   for (int m = 0; m < nmaps; ++m) etot += nelems[m];
   OP2HIP_REQUIRE(ninc <= 0x7fffffffLL && etot <= 0x7fffffffLL,
                  "sparsity of %lld incidences, beyond int32 offsets", ninc);
-  Work w = work_layout(nrows, ninc);
+  Work w = work_layout(nrows, ninc, cmax);
   OP2HIP_REQUIRE(work_bytes >= static_cast<long long>(w.bytes),
                  "sparsity workspace of %lld bytes, %zu needed", work_bytes, w.bytes);
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -515,7 +528,7 @@ int op2hip_sparsity_build(int rmult, int cmult, int nrows, int nmaps, const int*
     hipLaunchKernelGGL(scatter_incidences, dim3((nk + 255) / 256), dim3(256), 0, s, rowmaps[m],
                        nk, rdims[m], P.eoff[m], nrows, cnt, inc);
   }
-  RowOut o{nptr, rmult, cmult, nullptr, nullptr, ucnt, err};
+  RowOut o{nptr, rmult, cmult, nullptr, nullptr, ucnt, err, at<int>(base, w.uniq)};
   if (nrows > 0) {
     long long threads = static_cast<long long>(nrows) * 64;
     hipLaunchKernelGGL(rows_wave<false>, dim3((threads + 255) / 256), dim3(256), 0, s, iptr, inc,
@@ -567,7 +580,8 @@ int op2hip_sparsity_fill(void* handle, const int* rowptr, int* colidx, int* nnz_
   char* base = sb->work;
   const Work& w = sb->w;
   int* err = at<int>(base, w.err);
-  RowOut o{at<int>(base, w.nptr), sb->rmult, sb->cmult, colidx, nnz_rows, nullptr, err};
+  RowOut o{at<int>(base, w.nptr), sb->rmult, sb->cmult, colidx, nnz_rows, nullptr, err,
+           at<int>(base, w.uniq)};
   if (sb->nrows > 0) {
     long long threads = static_cast<long long>(sb->nrows) * 64;
     hipLaunchKernelGGL(rows_wave<true>, dim3((threads + 255) / 256), dim3(256), 0, s,

@@ -64,18 +64,21 @@ class _validate_base(object):
 
         # (name, position, has default, default, condition, exception) of
         # each check on a parameter the function has, resolved once
-        checks = [(a, names.index(a), a in defaults, defaults.get(a), cond, exc)
+        checks = [(a, names.index(a), a in defaults, defaults.get(a), cond, exc,
+                   self.quick(cond))
                   for a, cond, exc in self._checks if a in names]
         check_arg = self.check_arg
 
         @functools.wraps(f)
         def wrapper(*args, **kwargs):
-            for argname, i, has_default, default, cond, exc in checks:
+            for argname, i, has_default, default, cond, exc, quick in checks:
                 if argname in kwargs:
                     arg = kwargs[argname]
                 elif i < len(args):
                     arg = args[i]
                 else:
+                    continue
+                if quick(arg):
                     continue
                 if has_default:
                     try:
@@ -88,8 +91,17 @@ class _validate_base(object):
         return wrapper
 
 
+    def quick(self, cond):
+        """A predicate accepting (some) valid arguments cheaply; arguments
+        it does not accept go through the full check."""
+        return lambda arg: False
+
+
 class validate_type(_validate_base):
     """Check that named arguments have the given types."""
+
+    def quick(self, argtype):
+        return lambda arg: isinstance(arg, argtype)
 
     def check_arg(self, arg, argtype, exception, where):
         if not isinstance(arg, argtype):
@@ -99,6 +111,10 @@ class validate_type(_validate_base):
 
 class validate_in(_validate_base):
     """Check that named arguments are in a list of legal values."""
+
+    def quick(self, values):
+        ids = frozenset(id(v) for v in values)
+        return lambda arg: id(arg) in ids
 
     def check_arg(self, arg, values, exception, where):
         if not any(arg is v or arg == v for v in values):
