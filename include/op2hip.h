@@ -109,6 +109,29 @@ int op2hip_plan_verify(int nelems, int nblocks, const int* elems, const int* off
                        const int* const* maps, const int* dims, const int* idxs,
                        const int* spaces, int nspaces, const int* space_size, int* ok);
 
+/* The coloured plan built on the device (the backend's default; the host
+ * build above is kept as its alternative).  mode 0: two-level colouring of
+ * blocks of part_size consecutive elements (block colours, then thread
+ * colours inside each block greedily in element order, as plan.pyx); mode
+ * 2: element colours, each colour's elements cut into blocks.  Colours by
+ * Jones-Plassmann-Luby rounds with fixed hash priorities (deterministic).
+ * maps: host array of device int32 map pointers; the other arguments as
+ * op2hip_plan_build's.  work: op2hip_plan_workspace() bytes kept until
+ * fill or discard; thrcol (device, nelems) receives mode 0's thread
+ * colours.  dbuild returns nblocks and ncolors; dfill writes elems and
+ * thrcol (device, nelems) and offset, nelem, nthrcol, blkcol, blkmap
+ * (host, nblocks) and ncolblk (host, ncolors) with the meaning of
+ * op2hip_plan_array's, and releases the handle.  Both wait for the stream. */
+long long op2hip_plan_workspace(int nelems, int part_size, int nac, const int* space_size,
+                                int nspaces);
+int op2hip_plan_dbuild(int nelems, int part_size, int mode, int nmaps, const int* const* maps,
+                       const int* dims, const int* idxs, const int* spaces, int nspaces,
+                       const int* space_size, void* work, long long work_bytes, int* thrcol,
+                       void** handle, int* nblocks_out, int* ncolors_out, void* stream);
+int op2hip_plan_dfill(void* handle, int* elems, int* thrcol, int* offset, int* nelem,
+                      int* nthrcol, int* blkcol, int* blkmap, int* ncolblk, void* stream);
+int op2hip_plan_ddiscard(void* handle);
+
 /* Owner-computes plan for loops whose conflicting writes are increments into
  * one target set (also replaces pyop2/plan.pyx:68-456 for those loops).
  * Targets are owned by blocks in contiguous ranges of k; a block runs every

@@ -83,6 +83,12 @@ _SIGS = {
     'op2hip_plan_verify': (c_int, [c_int, c_int, c_void, c_void, c_void, c_void, c_void, c_int,
                                    P(c_void), P(c_int), P(c_int), P(c_int), c_int, P(c_int),
                                    P(c_int)]),
+    'op2hip_plan_workspace': (c_ll, [c_int, c_int, c_int, P(c_int), c_int]),
+    'op2hip_plan_dbuild': (c_int, [c_int, c_int, c_int, c_int, P(c_void), P(c_int), P(c_int),
+                                   P(c_int), c_int, P(c_int), c_void, c_ll, c_void, P(c_void),
+                                   P(c_int), P(c_int), c_void]),
+    'op2hip_plan_dfill': (c_int, [c_void] * 9 + [c_void]),
+    'op2hip_plan_ddiscard': (c_int, [c_void]),
     'op2hip_owner_plan_build': (c_int, [c_int, c_int, c_int, P(c_void), P(c_int), P(c_int), c_int,
                                         c_void, P(c_void)]),
     'op2hip_owner_plan_scalar': (c_int, [c_void, c_int, P(ctypes.c_int64)]),

@@ -15,7 +15,8 @@ LIBDIR = os.path.join(HERE, 'lib')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('OP2HIP_ARCH', 'gfx950')
 
-SOURCES = ['runtime.hip', 'plan.cpp', 'plan_device.hip', 'sparsity.hip', 'linalg.hip',
+SOURCES = ['runtime.hip', 'plan.cpp', 'plan_device.hip', 'plan_colour.hip', 'sparsity.hip',
+           'linalg.hip',
            'cg_batch.hip', 'krylov.hip']
 
 # Device code is built without packed FP32 instructions (some of their forms

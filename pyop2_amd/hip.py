@@ -612,6 +612,69 @@ class Plan(object):
         return bool(ok.value)
 
 
+class DevicePlan(Plan):
+    """A coloured plan built on the device (csrc/plan_colour.hip:
+    op2hip_plan_dbuild/dfill): mode 0 colours blocks of ``part_size``
+    consecutive elements and threads inside them (the reference's two
+    levels), mode 2 colours elements and cuts each colour into blocks.
+    ``race`` as for :class:`Plan`, with the hip Maps (their device values)
+    first: [(map, dim, idx, space, space_size)].  The arrays are fetched to
+    the host (the attributes of :class:`Plan`), where :meth:`verify` checks
+    them."""
+
+    def __init__(self, n, part_size, mode, race):
+        self.n = n
+        self.part_size = part_size
+        spaces = []
+        for r in race:
+            if r[3] not in spaces:
+                spaces.append(r[3])
+        sizes = [0] * len(spaces)
+        for r in race:
+            sizes[spaces.index(r[3])] = r[4]
+        self.race = [(r[0]._layout_values()[:n], r[1], r[2], r[3], r[4]) for r in race]
+        self._maps_keep = [np.ascontiguousarray(r[0], dtype=np.int32) for r in self.race]
+        self._args = (nat.ptr_array([m.ctypes.data for m in self._maps_keep]),
+                      nat.int_array([r[1] for r in race]),
+                      nat.int_array([r[2] for r in race]),
+                      nat.int_array([spaces.index(r[3]) for r in race]), len(spaces),
+                      nat.int_array(sizes))
+        _, dims, idxs, sp, nsp, ssz = self._args
+        nac = sum(r[1] if r[2] < 0 else 1 for r in race)
+        wbytes = nat.lib.op2hip_plan_workspace(n, part_size, nac, ssz, nsp)
+        if wbytes < 0:
+            raise RuntimeError("bad device plan arguments")
+        work = _dev_bytes(wbytes)
+        elems = torch.empty(max(n, 1), dtype=torch.int32, device=device())
+        thrcol = torch.empty(max(n, 1), dtype=torch.int32, device=device())
+        dmaps = [r[0]._device_values() for r in race]
+        h = ctypes.c_void_p()
+        nb, nc = ctypes.c_int(), ctypes.c_int()
+        s = stream()
+        nat.check(nat.lib.op2hip_plan_dbuild(
+            n, part_size, mode, len(race), nat.ptr_array([_ptr(t) for t in dmaps]), dims, idxs,
+            sp, nsp, ssz, ctypes.c_void_p(_ptr(work)), int(work.numel()),
+            ctypes.c_void_p(_ptr(thrcol)), ctypes.byref(h), ctypes.byref(nb), ctypes.byref(nc),
+            ctypes.c_void_p(s)), 'device plan build')
+        self.nblocks, self.ncolors = nb.value, nc.value
+        small = {k: np.zeros(self.nblocks, dtype=np.int32)
+                 for k in ('offset', 'nelems', 'nthrcol', 'blkcol', 'blkmap')}
+        small['ncolblk'] = np.zeros(self.ncolors, dtype=np.int32)
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        nat.check(nat.lib.op2hip_plan_dfill(
+            h, ctypes.c_void_p(_ptr(elems)), ctypes.c_void_p(_ptr(thrcol)), P(small['offset']),
+            P(small['nelems']), P(small['nthrcol']), P(small['blkcol']), P(small['blkmap']),
+            P(small['ncolblk']), ctypes.c_void_p(s)), 'device plan fill')
+        del work
+        for k, v in small.items():
+            setattr(self, k, v)
+        self.elems = elems[:n].cpu().numpy().astype(np.int32)
+        self.thrcol = thrcol[:n].cpu().numpy().astype(np.int32)
+        self.nthrcol_max = int(self.nthrcol.max()) if self.nblocks else 0
+        self.colour_start = np.concatenate([[0], np.cumsum(self.ncolblk)]).astype(np.int64)
+        self._dev = None
+
+
 _plan_cache = {}
 
 
@@ -652,11 +715,26 @@ def get_plan(iterset, n, args, part_size=None, reorder=None):
         if k not in seen:
             seen.add(k)
             uniq.append(r)
-    key = (id(iterset), n, part_size, int(reorder), tuple((id(r[0]), r[1], r[2]) for r in uniq))
+    dev = bool(cfg.get('hip_device_plan', True))
+    key = (id(iterset), n, part_size, int(reorder), dev,
+           tuple((id(r[0]), r[1], r[2]) for r in uniq))
     p = _plan_cache.get(key)
+    if p is None and dev:
+        # on the device: element colours (reorder 2), or the two levels over
+        # blocks of consecutive elements of the device layout (reorder 0 and
+        # 1: the layout already orders the iteration set into compact runs)
+        try:
+            p = DevicePlan(n, part_size, 2 if int(reorder) == 2 else 0,
+                           [(r[0], r[0].dim, r[1], r[2], r[3]) for r in uniq])
+        except nat.NativeError as e:
+            # more colours than the device colouring keeps (a target shared
+            # by hundreds of elements): the host build colours in rounds
+            debug("device plan refused (%s); building on the host" % e)
+            p = None
     if p is None:
         p = Plan(n, part_size, reorder,
                  [(r[0]._layout_values()[:n], r[0].dim, r[1], r[2], r[3]) for r in uniq])
+    if key not in _plan_cache:
         p._keep = [r[0] for r in uniq]
         _plan_cache[key] = p
     return p

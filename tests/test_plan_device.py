@@ -128,3 +128,78 @@ def test_mat_program_device_matches_host(hip, dim, k):
     assert np.array_equal(poff_d.cpu().numpy(), poff_h)
     pd = prog_d.cpu().numpy().view(np.int32 if wide else np.uint16)[:len(prog_h)]
     assert np.array_equal(pd, prog_h)
+
+
+def _coloured(op2, vals_spec, ntargets_by_space, n, ps, mode):
+    """A device coloured plan over hip Maps with identity layouts;
+    vals_spec: [(values, idx, space)]."""
+    from pyop2_amd import hip as H, layout
+    src = op2.Set(n, 1)
+    layout._group(src).layout = layout.IDENTITY
+    dsts, race = {}, []
+    for v, idx, sp in vals_spec:
+        if sp not in dsts:
+            dsts[sp] = op2.Set(ntargets_by_space[sp], 1)
+            layout._group(dsts[sp]).layout = layout.IDENTITY
+        m = op2.Map(src, dsts[sp], v.shape[1], v)
+        race.append((m, v.shape[1], idx, sp, ntargets_by_space[sp]))
+    return H.DevicePlan(n, ps, mode, race)
+
+
+def _check_coloured(p, n):
+    assert p.verify()
+    assert sorted(p.elems.tolist()) == list(range(n))
+    assert p.nelems.sum() == n and p.ncolblk.sum() == p.nblocks
+    assert sorted(p.blkmap.tolist()) == list(range(p.nblocks))
+    for b in range(p.nblocks):
+        assert p.offset[b] + p.nelems[b] <= n
+    # blocks in blkmap order run colour by colour
+    assert (np.diff(p.blkcol[p.blkmap]) >= 0).all()
+
+
+@pytest.mark.parametrize('mode', [0, 2])
+@pytest.mark.parametrize('seed', range(3))
+def test_device_coloured_plan_random(hip, mode, seed):
+    rng = np.random.default_rng(seed)
+    n, nt = 6000, 1500
+    v = rng.integers(0, nt, size=(n, 3)).astype(np.int32)
+    w = rng.integers(0, 700, size=(n, 2)).astype(np.int32)
+    spec = [(v, -1, 'a')] if seed == 0 else [(v, 0, 'a'), (v, 2, 'a'), (w, 1, 'b')]
+    p = _coloured(hip, spec, {'a': nt, 'b': 700}, n, 128 if seed != 2 else 512, mode)
+    _check_coloured(p, n)
+    if mode == 2:
+        assert (p.nthrcol == 1).all() and (p.thrcol == 0).all()
+
+
+@pytest.mark.parametrize('mode', [0, 2])
+def test_device_coloured_plan_meshes(hip, mode):
+    m = meshes.QuadMesh(60, 30, seed=5)
+    p = _coloured(hip, [(m.pecell, 0, 'c'), (m.pecell, 1, 'c')], {'c': m.ncell}, m.nedge, 256,
+                  mode)
+    _check_coloured(p, m.nedge)
+    t = meshes.TetMesh(7, seed=3)
+    p = _coloured(hip, [(t.elem_node.astype(np.int32), -1, 'n')], {'n': t.nvert}, t.nelem, 256,
+                  mode)
+    _check_coloured(p, t.nelem)
+
+
+def test_device_coloured_plan_fan_falls_back(hip):
+    """A target shared by 3000 elements needs more element colours than the
+    device colouring keeps: the plan is built on the host instead; the
+    two-level form colours its blocks and threads on the device."""
+    from pyop2_amd import hip as H, layout, configuration as cfg
+    n = 3000
+    k_ = np.arange(1, n + 1)
+    fan = np.stack([np.zeros(n, dtype=np.int64), k_, k_ + 1], axis=1).astype(np.int32)
+    p = _coloured(hip, [(fan, -1, 'f')], {'f': n + 2}, n, 256, 0)
+    _check_coloured(p, n)
+    src = hip.Set(n, 1)
+    dst = hip.Set(n + 2, 1)
+    layout._group(src).layout = layout.IDENTITY
+    layout._group(dst).layout = layout.IDENTITY
+    m = hip.Map(src, dst, 3, fan)
+    d = hip.Dat(dst, np.zeros(n + 2), np.float64)
+    args = [d(m[hip.i[0]], hip.INC)]
+    with cfg.override(hip_colouring='elements'):
+        q = H.get_plan(src, n, args, part_size=256)
+    assert not isinstance(q, H.DevicePlan) and q.verify()
