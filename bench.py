@@ -168,7 +168,9 @@ def main():
     step_bytes = ub['save_soln'] + 2 * (ub['adt_calc'] + ub['res_calc'] + ub['bres_calc'] +
                                         ub['update'])
     s = hip.stream()
-    for _ in range(args.warmup):
+    # a captured step needs one eager step before it (compilation, plans,
+    # uploads)
+    for _ in range(max(args.warmup, 1 if args.graph else 0)):
         A.iteration()
     torch.cuda.synchronize()
 

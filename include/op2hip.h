@@ -204,6 +204,29 @@ int op2hip_mat_program_device(int nactive, const int* blk_t0, const int* blk_t1,
                               long long npairs, int wide, void* work, long long work_bytes,
                               int* poff, void* prog, void* stream);
 
+/* Row-block plan of the owner Mat "rows" template (replaces, for Mat
+ * assembly, the plan of plan.pyx:68-456 and the per-entry search of
+ * mat_utils.cxx:4-28 / petsc_base.py:92-98).  Block b owns row nodes
+ * [b*k, min(nnodes, (b+1)*k)) and runs the elements naming them, in
+ * ascending user order (rank; NULL: identity); one thread per owned node
+ * adds its records' entries into the block's CSR slice in sequential order.
+ * Outputs (device, caller-allocated): tp[nnodes+1] record offsets per node,
+ * xoff[nblk+1] and exec (nelems*e0, an upper bound) the blocks' elements,
+ * code32[nelems*e0] codes slot*e0 + i0, pos[nelems*e0*ceil(e1/4)] column
+ * positions as bytes.  scalars (host, 5): largest code, largest exec list,
+ * largest CSR slice (entries), exec entries in all, flags (1: an entry
+ * outside the pattern, 2: a row of more than 256 node columns, 4: an
+ * inconsistent plan).  work: op2hip_mat_rows_workspace() bytes.  Waits for
+ * the stream once. */
+long long op2hip_mat_rows_workspace(int nelems, int nnodes, int e0, int k);
+int op2hip_mat_rows_build(int nelems, int nnodes, const int* rmap, int rarity, int e0,
+                          const int* cmap, int carity, int e1, const int* rank,
+                          const int* rowptr, const int* colidx, int rd, int cd, int k, void* work,
+                          long long work_bytes, int* tp, int* xoff, int* exec, int* code32,
+                          unsigned* pos, long long* scalars, void* stream);
+/* code32[0, n) narrowed to uint16 into out (every code below 65536). */
+int op2hip_mat_rows_pack16(const int* code32, long long n, unsigned short* out, void* stream);
+
 /* ---- data layout ----------------------------------------------------------------
  * No counterpart in the reference, which stores every Set in user order.
  * Orderings that renumber a Set's device storage for locality (user-visible

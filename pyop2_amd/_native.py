@@ -106,6 +106,11 @@ _SIGS = {
                                                                   c_int, c_int, c_ll, c_int,
                                                                   c_void, c_ll, c_void, c_void,
                                                                   c_void]),
+    'op2hip_mat_rows_workspace': (c_ll, [c_int, c_int, c_int, c_int]),
+    'op2hip_mat_rows_build': (c_int, [c_int, c_int, c_void, c_int, c_int, c_void, c_int, c_int,
+                                      c_void, c_void, c_void, c_int, c_int, c_int, c_void, c_ll,
+                                      c_void, c_void, c_void, c_void, c_void, P(c_ll), c_void]),
+    'op2hip_mat_rows_pack16': (c_int, [c_void, c_ll, c_void, c_void]),
     'op2hip_mat_program': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void, c_void,
                                    c_void, c_int,
                                    c_void, c_int, c_int, c_int, c_void, c_void, c_int, c_int,

@@ -169,7 +169,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
 
     def __init__(self, kernel, extents, args, consts, mode, block_size, prefetch=1,
                  stage_reads=True, stage_writes=True, unroll=1, owner_stage=()):
-        assert mode in ('stream', 'coloured', 'owner')
+        assert mode in ('stream', 'coloured', 'owner', 'rows')
         # owner write-back: 1 issues the loads locating a thread's first
         # target before the elements run (they overlap), 0 after them; 2
         # also loads that target's old value early
@@ -225,6 +225,11 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                       ('own_k', None),
                       ('own_cpitch', None), ('own_poff', None), ('own_prog', None),
                       ('own_wide', None)]
+        elif self.mode == 'rows':
+            roles += [('own_tp', None), ('own_xoff', None), ('own_exec', None),
+                      ('own_contrib', None), ('own_c16', None), ('own_pos', None),
+                      ('own_k', None), ('own_nnodes', None), ('own_slice', None)]
+            roles += [('rows_xmap', m) for m in self.rows_maps()]
         for a in self.infos:
             for _, role in a.params():
                 roles.append((role, a.pos))
@@ -289,7 +294,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             lines.append('}')
         return lines
 
-    def _element_body(self, sfx='', ev='e', split=False, infos=None, kernel=None):
+    def _element_body(self, sfx='', ev='e', split=False, infos=None, kernel=None, mslot=None):
         """Code that runs the kernel for element ``ev`` (all itspace points),
         leaving INC increments and Mat entries in private arrays; private
         names carry the suffix ``sfx``.
@@ -300,8 +305,16 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         With ``split`` it returns (decls, loads, compute): ``loads`` copies
         the rows of read-only Dats into registers (local rows), so the
         template can issue the loads of several elements before their
-        kernels run."""
+        kernels run.  With ``split='maps'`` it returns (decls, mloads, loads,
+        compute): ``mloads`` copies the map entries of the local rows of
+        ind/vec arguments into private ints (``mi<pos>``, whatever ``sfx``),
+        which ``loads`` then reads, so a template can load an element's map
+        entries one step ahead of its rows; with ``mslot`` (an expression)
+        ``mloads`` reads them from the exec-ordered map rows ``xm<pos>`` at
+        that slot instead of the map at the element."""
         decls, loads, compute, call_args, zeros, post = [], [], [], [], [], []
+        mloads, mseen = [], set()
+        by_maps = split == 'maps'
         ext = self.extents
         infos = self.infos if infos is None else infos
         kernel = self.kernel if kernel is None else kernel
@@ -384,6 +397,18 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                                  % (self._m(p), E, a.mdim, a.idx, t, self._stage_of[p], a.cdim, p,
                                     a.cdim, a.cdim, n('rl')))
                     call_args.append(n('rl'))
+                elif local and by_maps:
+                    mi = 'mi%d_%d' % (self._m(p), a.idx)
+                    if mi not in mseen:
+                        mseen.add(mi)
+                        decls.append('int %s = 0;' % mi)
+                        mloads.append('%s = %s[%s * %d + %d];'
+                                      % (mi, *self._mrow(p, E, mslot), a.mdim, a.idx))
+                    decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
+                    loads.append('{ const %s* src = d%d + (op2_i64)%s * %d; '
+                                 'for (int k = 0; k < %d; ++k) %s[k] = src[k]; }'
+                                 % (t, p, mi, a.cdim, a.cdim, n('rl')))
+                    call_args.append(n('rl'))
                 elif local:
                     decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
                     loads.append('{ const %s* src = d%d + (op2_i64)m%d[%s * %d + %d] * %d; '
@@ -401,19 +426,37 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                                  'for (int k = 0; k < %d; ++k) %s[j][k] = (%s)0; }'
                                  % (a.mdim, n('v'), n('inc'), a.cdim, n('inc'), t))
                 elif a.soa:
+                    # the row pointers are set next to the call, so private
+                    # rows loaded one loop iteration ahead stay in registers
                     decls.append('%s %s[%d][%d];' % (t, n('rl'), a.mdim, a.cdim))
                     (loads if local else compute).append(
                         'for (int j = 0; j < %d; ++j) { const op2_i64 tt = m%d[%s * %d + j]; '
-                        '%s[j] = %s[j]; for (int k = 0; k < %d; ++k) %s[j][k] = d%d[%s]; }'
-                        % (a.mdim, self._m(p), E, a.mdim, n('v'), n('rl'), a.cdim, n('rl'), p,
-                           a.at('tt', 'k')))
+                        'for (int k = 0; k < %d; ++k) %s[j][k] = d%d[%s]; }'
+                        % (a.mdim, self._m(p), E, a.mdim, a.cdim, n('rl'), p, a.at('tt', 'k')))
+                    compute.append('for (int j = 0; j < %d; ++j) %s[j] = %s[j];'
+                                   % (a.mdim, n('v'), n('rl')))
+                elif local and by_maps:
+                    mi = 'mi%d' % self._m(p)
+                    if mi not in mseen:
+                        mseen.add(mi)
+                        decls.append('int %s[%d] = {0};' % (mi, a.mdim))
+                        mloads.append('for (int j = 0; j < %d; ++j) %s[j] = %s[%s * %d + j];'
+                                      % (a.mdim, mi, *self._mrow(p, E, mslot), a.mdim))
+                    decls.append('%s %s[%d][%d];' % (t, n('rl'), a.mdim, a.cdim))
+                    loads.append('for (int j = 0; j < %d; ++j) { const %s* src = d%d + '
+                                 '(op2_i64)%s[j] * %d; for (int k = 0; k < %d; ++k) '
+                                 '%s[j][k] = src[k]; }'
+                                 % (a.mdim, t, p, mi, a.cdim, a.cdim, n('rl')))
+                    compute.append('for (int j = 0; j < %d; ++j) %s[j] = %s[j];'
+                                   % (a.mdim, n('v'), n('rl')))
                 elif local:
                     decls.append('%s %s[%d][%d];' % (t, n('rl'), a.mdim, a.cdim))
                     loads.append('for (int j = 0; j < %d; ++j) { const %s* src = d%d + '
-                                 '(op2_i64)m%d[%s * %d + j] * %d; %s[j] = %s[j]; '
+                                 '(op2_i64)m%d[%s * %d + j] * %d; '
                                  'for (int k = 0; k < %d; ++k) %s[j][k] = src[k]; }'
-                                 % (a.mdim, t, p, self._m(p), E, a.mdim, a.cdim, n('v'), n('rl'),
-                                    a.cdim, n('rl')))
+                                 % (a.mdim, t, p, self._m(p), E, a.mdim, a.cdim, a.cdim, n('rl')))
+                    compute.append('for (int j = 0; j < %d; ++j) %s[j] = %s[j];'
+                                   % (a.mdim, n('v'), n('rl')))
                 else:
                     compute.append('for (int j = 0; j < %d; ++j) %s[j] = d%d + '
                                    '(op2_i64)m%d[%s * %d + j] * %d;'
@@ -468,9 +511,30 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         compute += ['}'] * len(opens)
         compute += closing
         compute += post
+        if by_maps:
+            return decls, mloads, loads, compute
         if split:
             return decls, loads, compute
         return decls, loads + compute
+
+    def _mrow(self, p, E, mslot):
+        """(array, row) of argument ``p``'s map entries: the map at element
+        ``E``, or the exec-ordered map rows at slot ``mslot``."""
+        if mslot is None:
+            return 'm%d' % self._m(p), E
+        return 'xm%d' % self._m(p), '(op2_i64)(%s)' % mslot
+
+    def rows_maps(self):
+        """Map parameter positions whose entries the rows template reads
+        from exec-ordered map rows (the maps of register rows)."""
+        mloads = self._element_body(split='maps', mslot='0')[1]
+        out = []
+        for l in mloads:
+            i = l.index('= xm') + 4
+            m = int(l[i:].split('[')[0])
+            if m not in out:
+                out.append(m)
+        return out
 
     def _m(self, p):
         """Position of the map parameter argument ``p`` reads."""
@@ -909,6 +973,206 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                ind(self._red_finish('b'), 2)]
         return out
 
+    # records a row thread holds in registers per reload
+    ROWS_RECORDS = 8
+
+    def _rows_body(self):
+        """Owner-computes Mat assembly by row blocks: block b owns the row
+        nodes [b*k, (b+1)*k) and runs the elements naming one of them in
+        ascending user order (its exec list), a chunk of one element per
+        thread at a time, keeping the chunk's local matrices in LDS.  After
+        each chunk, one thread per owned node adds the chunk's entries of its
+        row into the block's CSR slice, which sits in LDS from the first
+        chunk to the last: its records (element slot and row position, and
+        the column positions of the element's columns in the row) come in
+        ascending (user element, row position) order and each record's
+        columns in ascending order, so every entry receives its terms in the
+        order of the reference's sequential loop (addto_vector /
+        addto_scalar).  Element loads run as a pipeline over the chunks:
+        map entries two chunks ahead (from the plan's exec-ordered copy of
+        the map rows: coalesced, no element number needed first) and rows
+        one ahead, into a second set of registers (the chunk loop is
+        unrolled by two, the sets trading roles).  No atomics, no colours;
+        the slice is read and written once."""
+        bs = self.bs
+        ml = lambda slot: self._element_body(split='maps', mslot=slot)[1]
+        decls, _, loads, compute = self._element_body(split='maps', mslot='0')
+        decls_n, _, loads_n, compute_n = self._element_body(sfx='_n', ev='e_n', split='maps',
+                                                            mslot='0')
+        decls_n = [l for l in decls_n if not l.startswith('int mi')]
+        ahead = ml('ex0 + c0 + 2 * %d + threadIdx.x' % bs)
+        ind = lambda lines, n: '\n'.join(' ' * n + l for l in lines)
+        if self.reductions:
+            raise NotImplementedError("row-block Mat assembly with Global reductions")
+        a = [x for x in self.infos if x.kind == 'mat'][0]
+        p, bs, t = a.pos, self.bs, a.ctype
+        e0, e1 = self.extents[0], self.extents[1]
+        rd, cd = a.rdim, a.cdim_m
+        W = (e1 + 3) // 4
+        sub = {'p': p, 'bs': bs, 't': t, 'e0': e0, 'e1': e1, 'rd': rd, 'cd': cd, 'W': W,
+               'ent': e0 * e1 * rd * cd, 'P': self.ROWS_RECORDS,
+               'pitch': self.rows_pitch(),
+               'op': '=' if a.access is WRITE else '+='}
+        head = """  extern __shared__ double op2_lds[];
+  // consecutive blocks (neighbouring row ranges) go to one XCD
+  const int nb = (int)count, bid = blockIdx.x;
+  const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
+  const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
+  const int kk = (int)own_k, t0 = b * kk;
+  const int t1 = t0 + kk < (int)own_nnodes ? t0 + kk : (int)own_nnodes, nr = t1 - t0;
+  const int ex0 = p_xoff[b], nex = p_xoff[b + 1] - ex0;
+  const int base = rp%(p)d[(op2_i64)t0 * %(rd)d];
+  const int nsl = rp%(p)d[(op2_i64)t1 * %(rd)d] - base;
+  // LDS: the block's CSR slice [own_slice], the chunk's local matrices
+  // [%(bs)d][%(pitch)d] (an odd pitch: a row thread reads the entries of one
+  // local-matrix row together, and the slots its wavefront reads spread
+  // over the banks) and the owned scalar rows' offsets in the slice
+  // [k * %(rd)d + 1]
+  %(t)s* s_acc = (%(t)s*)op2_lds;
+  %(t)s* s_val = s_acc + (op2_i64)own_slice;
+  int* s_rp = (int*)(s_val + %(pitch)d * %(bs)d);
+  for (int i = threadIdx.x; i < nsl; i += %(bs)d) s_acc[i] = d%(p)d[(op2_i64)base + i];
+  for (int i = threadIdx.x; i <= nr * %(rd)d; i += %(bs)d)
+    s_rp[i] = rp%(p)d[(op2_i64)t0 * %(rd)d + i] - base;
+  // a row thread's records: [rec, rend)
+  int rec = 0, rend = 0;
+  if ((int)threadIdx.x < nr) {
+    rec = p_tp[t0 + threadIdx.x];
+    rend = p_tp[t0 + threadIdx.x + 1];
+  }
+  // the element pipeline's loads take no guard (the exec list and the
+  // exec-ordered map rows carry padding naming element 0, whose map
+  // entries and rows exist), so they stay straight-line code whose
+  // outstanding loads the compiler counts exactly; elements past nex are
+  // loaded but never run
+  int e = p_exec[ex0 + threadIdx.x];
+  int e_n = p_exec[ex0 + %(bs)d + threadIdx.x];""" % sub
+        # a record (code cv: slot * e0 + i0; column positions in words qv)
+        # added into the row
+        apply = """{
+  const int sl = CV / %(e0)d, i0 = CV - sl * %(e0)d, cl = sl - c0;
+  int kc[%(e1)d];
+  %(t)s v[%(e1)d][%(rd)d][%(cd)d];
+  bool dup = false;
+#pragma unroll
+  for (int i1 = 0; i1 < %(e1)d; ++i1) {
+    kc[i1] = (int)((QV[i1 >> 2] >> (8 * (i1 & 3))) & 255u);
+#pragma unroll
+    for (int j1 = 0; j1 < i1; ++j1) dup |= kc[j1] == kc[i1];
+#pragma unroll
+    for (int r = 0; r < %(rd)d; ++r)
+#pragma unroll
+      for (int c = 0; c < %(cd)d; ++c)
+        v[i1][r][c] = s_val[cl * %(pitch)d + ((i0 * %(e1)d + i1) * %(rd)d + r) * %(cd)d + c];
+  }
+  if (!dup) {
+    // distinct entries: their loads issue together
+    %(t)s w[%(e1)d][%(rd)d][%(cd)d];
+#pragma unroll
+    for (int i1 = 0; i1 < %(e1)d; ++i1)
+#pragma unroll
+      for (int r = 0; r < %(rd)d; ++r)
+#pragma unroll
+        for (int c = 0; c < %(cd)d; ++c) w[i1][r][c] = s_acc[rows[r] + kc[i1] * %(cd)d + c];
+#pragma unroll
+    for (int i1 = 0; i1 < %(e1)d; ++i1)
+#pragma unroll
+      for (int r = 0; r < %(rd)d; ++r)
+#pragma unroll
+        for (int c = 0; c < %(cd)d; ++c) {
+          w[i1][r][c] %(op)s v[i1][r][c];
+          s_acc[rows[r] + kc[i1] * %(cd)d + c] = w[i1][r][c];
+        }
+  } else {
+    // an entry named twice (a node repeated among the columns): one
+    // column after the other
+#pragma unroll
+    for (int i1 = 0; i1 < %(e1)d; ++i1)
+#pragma unroll
+      for (int r = 0; r < %(rd)d; ++r)
+#pragma unroll
+        for (int c = 0; c < %(cd)d; ++c)
+          s_acc[rows[r] + kc[i1] * %(cd)d + c] %(op)s v[i1][r][c];
+  }
+}""" % sub
+        fast = apply.replace('CV', 'rc[u]').replace('QV', 'rq[u]').split('\n')
+        slow = apply.replace('CV', 'cv').replace('QV', 'qv').split('\n')
+        fetch = """    // the row thread's next %(P)d records (the record arrays carry padding
+    // entries: these loads need no guard and issue together; past rend a
+    // code reads as INT_MAX), in flight while the chunk's elements run
+    int rc[%(P)d];
+    unsigned rq[%(P)d][%(W)d];
+#pragma unroll
+    for (int u = 0; u < %(P)d; ++u) {
+      const int cv = (int)((const unsigned short*)p_contrib)[rec + u];
+      rc[u] = rec + u < rend ? cv : 0x7fffffff;
+#pragma unroll
+      for (int w = 0; w < %(W)d; ++w) rq[u][w] = p_pos[(op2_i64)(rec + u) * %(W)d + w];
+    }""" % sub
+        rows = """    __syncthreads();
+    if ((int)threadIdx.x < nr) {
+      int rows[%(rd)d];
+#pragma unroll
+      for (int r = 0; r < %(rd)d; ++r) rows[r] = s_rp[threadIdx.x * %(rd)d + r];
+      // this chunk's records have codes below cend
+      const int cend = (c0 + %(bs)d) * %(e0)d;
+      int dn = 0;
+#pragma unroll
+      for (int u = 0; u < %(P)d; ++u) {
+        if (dn == u && rc[u] < cend) {""" % sub
+        rows2 = """          ++dn;
+        }
+      }
+      rec += dn;
+      // more than %(P)d records in this chunk: the rest one at a time
+      while (dn == %(P)d && rec < rend) {
+        const int cv = (int)((const unsigned short*)p_contrib)[rec];
+        if (cv >= cend) break;
+        unsigned qv[%(W)d];
+#pragma unroll
+        for (int w = 0; w < %(W)d; ++w) qv[w] = p_pos[(op2_i64)rec * %(W)d + w];""" % sub
+        rows3 = """        ++rec;
+      }
+    }
+    __syncthreads();""" % sub
+
+        def step(ev, ldn, cmp, lm):
+            # one chunk: compute from one register set while the next chunk's
+            # rows load into the other and the map entries of the one after
+            # into mi
+            return ['    {', fetch, ind(ldn, 4), ind(ahead, 4),
+                    '    if (c0 + (int)threadIdx.x < nex) {', ind(cmp, 6),
+                    '#pragma unroll',
+                    '      for (int j = 0; j < %(ent)d; ++j)' % sub,
+                    '        s_val[threadIdx.x * %d + j] = ((const %s*)%s)[j];'
+                    % (sub['pitch'], t, lm),
+                    '    }',
+                    '    %s = p_exec[ex0 + c0 + 2 * %d + threadIdx.x];' % (ev, bs),
+                    rows, ind(fast, 10), rows2, ind(slow, 8), rows3, '    }']
+        out = [head, ind(decls, 2), ind(decls_n, 2), ind(ml('ex0 + threadIdx.x'), 2),
+               ind(loads, 2), ind(ml('ex0 + %d + threadIdx.x' % bs), 2),
+               '  for (int c0 = 0; c0 < nex; c0 += %(bs)d) {' % sub]
+        out += step('e', loads_n, compute, 'lm%d' % p)
+        out += ['    c0 += %(bs)d;' % sub, '    if (c0 >= nex) break;']
+        out += step('e_n', loads, compute_n, 'lm%d_n' % p)
+        out += ['  }',
+                '  for (int i = threadIdx.x; i < nsl; i += %(bs)d) d%(p)d[(op2_i64)base + i] = s_acc[i];'
+                % sub]
+        return out
+
+    def rows_pitch(self):
+        """Values per slot of a row block's local matrices (odd)."""
+        a = [x for x in self.infos if x.kind == 'mat'][0]
+        ent = self.extents[0] * self.extents[1] * a.rdim * a.cdim_m
+        return ent | 1
+
+    def rows_lds_bytes(self, k, slice_max):
+        """LDS bytes of a row block: its CSR slice, the chunk's local
+        matrices and its rows' slice offsets."""
+        a = [x for x in self.infos if x.kind == 'mat'][0]
+        isz = np.dtype(a.arg.data.dtype).itemsize
+        return isz * (slice_max + self.rows_pitch() * self.bs) + 4 * (k * a.rdim + 1)
+
     def _stage_offset(self, pos):
         """Values per owned row staged before argument ``pos`` (all of
         them for None)."""
@@ -946,6 +1210,12 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                      'op2_i64 own_c16', 'op2_i64 lds_pitch', 'op2_i64 own_k', 'op2_i64 own_cpitch',
                      'const int* __restrict__ p_poff', 'const void* __restrict__ p_prog',
                      'op2_i64 own_wide']
+        elif self.mode == 'rows':
+            decl += ['const int* __restrict__ p_tp', 'const int* __restrict__ p_xoff',
+                     'const int* __restrict__ p_exec', 'const void* __restrict__ p_contrib',
+                     'op2_i64 own_c16', 'const unsigned* __restrict__ p_pos', 'op2_i64 own_k',
+                     'op2_i64 own_nnodes', 'op2_i64 own_slice']
+            decl += ['const int* __restrict__ xm%d' % m for m in self.rows_maps()]
         for a in self.infos:
             decl += [d for d, _ in a.params()]
         if self.has_mats:
@@ -963,6 +1233,10 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         bounds = '%d, %d' % (bs, self.min_waves) if self.min_waves else '%d' % bs
         out.append('extern "C" __global__ void __launch_bounds__(%s) %s(\n    %s) {'
                    % (bounds, self.entry, ',\n    '.join(self._params_decl())))
+        if self.mode == 'rows':
+            out += self._rows_body()
+            out.append('}')
+            return '\n'.join(out)
         decls, compute = self._element_body()
         if self.mode == 'stream':
             out += self._stream_body(decls, compute)

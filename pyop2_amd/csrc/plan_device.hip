@@ -836,3 +836,329 @@ int op2hip_mat_program_device(int nactive, const int* blk_t0, const int* blk_t1,
 }
 
 }  // extern "C"
+
+// ---- row-block Mat plans ----------------------------------------------------
+//
+// The plan of the owner Mat "rows" template.  Block b owns the k row nodes
+// [b*k, min(nnodes, (b+1)*k)) of the device layout and runs every element
+// naming one of them as a row, in ascending user order (rank), in chunks of
+// a workgroup; after each chunk one thread per owned node adds the chunk's
+// entries of its row into the block's CSR slice, in its records' order.
+// Arrays:
+//
+//   tp[nnodes+1], code   per row node, its records in ascending (rank, row
+//                        position i0): code = slot * e0 + i0, slot the
+//                        element's position in its block's exec list
+//   pos[ninc * W]        per record, W = ceil(e1 / 4) words of e1 bytes:
+//                        byte j is the position of column node cmap[e][j]
+//                        among the node columns of row node n
+//   xoff[nblk+1], exec   per block, its elements (device numbering) in
+//                        ascending rank
+//
+// Steps: count records per node, scan (tp), scatter 64-bit keys (rank,
+// element * e0 + i0) and sort each node's keys (the owner-plan kernels
+// above); count the distinct blocks of each element, scan (xoff), scatter
+// (rank, element) keys per block and sort each block's keys (LDS); then one
+// thread per node writes its records' codes and column positions.  The one
+// host synchronisation reads four scalars and the error flags.
+
+namespace {
+
+constexpr int kCapX = 4096;  // exec keys a workgroup sorts in LDS
+
+// Distinct blocks owning a row node of element e: count them (COUNT) or
+// scatter the key (rank << 32 | e) to each block's cursor.
+template <bool COUNT>
+__global__ void rows_exec_entries(const int* __restrict__ rmap, int rarity, int e0, int n,
+                                  int nnodes, int k, const int* __restrict__ rank,
+                                  int* __restrict__ cnt, u64* __restrict__ xkey) {
+  const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int* row = rmap + e * rarity;
+  for (int i = 0; i < e0; ++i) {
+    const int t = row[i];
+    if (static_cast<unsigned>(t) >= static_cast<unsigned>(nnodes)) continue;
+    const int b = t / k;
+    bool dup = false;
+    for (int j = 0; j < i; ++j) {
+      const int u = row[j];
+      dup |= static_cast<unsigned>(u) < static_cast<unsigned>(nnodes) && u / k == b;
+    }
+    if (dup) continue;
+    if (COUNT) {
+      atomicAdd(cnt + b, 1);
+    } else {
+      const unsigned r = rank ? static_cast<unsigned>(rank[e]) : static_cast<unsigned>(e);
+      const int at = atomicAdd(cnt + b, 1);
+      xkey[at] = (static_cast<u64>(r) << 32) | static_cast<unsigned>(e);
+    }
+  }
+}
+
+// One workgroup per block: sort its exec keys (LDS up to kCapX keys, global
+// memory beyond) and write the element numbers.
+__global__ void __launch_bounds__(kThreads) rows_sort_exec(const int* __restrict__ xoff, int nblk,
+                                                           u64* xkey, int* __restrict__ exec) {
+  __shared__ u64 s_k[kCapX];
+  for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const int q0 = xoff[b], len = xoff[b + 1] - q0;
+    u64* a = len <= kCapX ? s_k : xkey + q0;
+    if (len <= kCapX)
+      for (int i = threadIdx.x; i < len; i += kThreads) s_k[i] = xkey[q0 + i];
+    __syncthreads();
+    block_sort<u64>(a, len);
+    __syncthreads();
+    for (int i = threadIdx.x; i < len; i += kThreads) {
+      const u64 v = a[i];
+      xkey[q0 + i] = v;
+      exec[q0 + i] = static_cast<int>(static_cast<unsigned>(v & 0xffffffffull));
+    }
+    __syncthreads();
+  }
+}
+
+// Per block: the largest CSR slice and exec list (scal[2], scal[1]).
+__global__ void rows_stats(const int* __restrict__ rowptr, int nnodes, int rd, int k, int nblk,
+                           const int* __restrict__ xoff, int* __restrict__ scal) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const long long t0 = static_cast<long long>(b) * k;
+  const long long t1 = t0 + k < nnodes ? t0 + k : nnodes;
+  atomicMax(scal + 2, rowptr[t1 * rd] - rowptr[t0 * rd]);
+  atomicMax(scal + 1, xoff[b + 1] - xoff[b]);
+}
+
+// One thread per row node: its records' codes and column positions.  Flags
+// (scal[4]): 1 a column outside the pattern, 2 a position beyond 255, 4 an
+// element missing from its block's exec list.  scal[0]: the largest code.
+__global__ void rows_records(const u64* __restrict__ key, const int* __restrict__ tp, int nnodes,
+                             int e0, int k, const u64* __restrict__ xkey,
+                             const int* __restrict__ xoff, const int* __restrict__ cmap,
+                             int carity, int e1, const int* __restrict__ rowptr,
+                             const int* __restrict__ colidx, int rd, int cd, int W,
+                             int* __restrict__ code32, unsigned* __restrict__ pos,
+                             int* __restrict__ scal) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  const int c0 = tp[n], c1 = tp[n + 1];
+  if (c0 == c1) return;
+  const int b = n / k, x0 = xoff[b], x1 = xoff[b + 1];
+  const int s0 = rowptr[static_cast<long long>(n) * rd];
+  const int ncn = (rowptr[static_cast<long long>(n) * rd + 1] - s0) / cd;
+  int flags = 0, cmax = 0;
+  for (int c = c0; c < c1; ++c) {
+    const u64 kk = key[c];
+    const unsigned low = static_cast<unsigned>(kk & 0xffffffffull);
+    const int e = static_cast<int>(low / static_cast<unsigned>(e0));
+    const int i0 = static_cast<int>(low - static_cast<unsigned>(e) * e0);
+    const u64 want = (kk & 0xffffffff00000000ull) | static_cast<unsigned>(e);
+    int lo = x0, hi = x1 - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (xkey[mid] < want)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (lo > hi || xkey[lo] != want) flags |= 4;
+    const int code = (lo - x0) * e0 + i0;
+    code32[c] = code;
+    cmax = code > cmax ? code : cmax;
+    for (int w = 0; w < W; ++w) {
+      unsigned word = 0;
+      for (int j = 4 * w; j < 4 * w + 4 && j < e1; ++j) {
+        const int m = cmap[static_cast<long long>(e) * carity + j];
+        int a = 0, z = ncn - 1, at = -1;
+        while (a <= z) {
+          const int mid = (a + z) >> 1;
+          const int v = colidx[s0 + mid * cd] / cd;
+          if (v == m) {
+            at = mid;
+            break;
+          }
+          if (v < m)
+            a = mid + 1;
+          else
+            z = mid - 1;
+        }
+        if (at < 0) {
+          flags |= 1;
+          at = 0;
+        }
+        if (at > 255) {
+          flags |= 2;
+          at = 0;
+        }
+        word |= static_cast<unsigned>(at) << (8 * (j - 4 * w));
+      }
+      pos[static_cast<long long>(c) * W + w] = word;
+    }
+  }
+  atomicMax(scal, cmax);
+  if (flags) atomicOr(scal + 4, flags);
+}
+
+__global__ void rows_pack16(const int* __restrict__ c32, long long n,
+                            unsigned short* __restrict__ out) {
+  const long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = static_cast<unsigned short>(c32[q]);
+}
+
+struct RowsWork {
+  size_t cnt, key, longlist, nlong, sums, total, cntb, xkey, scal, bytes;
+};
+
+RowsWork rows_layout(long long nnodes, long long ninc, long long nblk) {
+  RowsWork w;
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    size_t at = o;
+    o += align256(b > 0 ? b : 1);
+    return at;
+  };
+  const size_t I = sizeof(int);
+  w.cnt = take(I * (nnodes + 1));
+  w.key = take(sizeof(u64) * ninc);
+  w.longlist = take(I * (nnodes + 1));
+  w.nlong = take(I);
+  long long ns = scan_tiles_for(nnodes + 1);
+  long long nb2 = scan_tiles_for(nblk + 1);
+  w.sums = take(sizeof(long long) * (ns > nb2 ? ns : nb2));
+  w.total = take(sizeof(long long) * 2);
+  w.cntb = take(I * (nblk + 1));
+  w.xkey = take(sizeof(u64) * ninc);
+  w.scal = take(I * 8);
+  w.bytes = o;
+  return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+long long op2hip_mat_rows_workspace(int nelems, int nnodes, int e0, int k) {
+  if (nelems < 0 || nnodes < 0 || e0 <= 0 || k <= 0) return -1;
+  const long long nblk = (static_cast<long long>(nnodes) + k - 1) / k;
+  return static_cast<long long>(
+      rows_layout(nnodes, static_cast<long long>(nelems) * e0, nblk).bytes);
+}
+
+// Build a row-block Mat plan on the device.  rmap / cmap: device row and
+// column maps (arities rarity / carity, device numbering), rows i0 < e0 and
+// columns j < e1 of each element; rank (device, may be NULL for the
+// identity) the user order of each element; rowptr / colidx: the device
+// CSR pattern with rd x cd blocks (scalar rows).  Outputs (device,
+// caller-allocated): tp (nnodes + 1), xoff (nblk + 1), exec and code32
+// (nelems * e0 each, an upper bound for exec), pos (nelems * e0 * W words,
+// W = ceil(e1 / 4)).  scalars (host, 5): the largest code, the largest exec
+// list, the largest CSR slice, the exec entries summed over blocks, and the
+// flags of rows_records.  Waits for the stream once.
+int op2hip_mat_rows_build(int nelems, int nnodes, const int* rmap, int rarity, int e0,
+                          const int* cmap, int carity, int e1, const int* rank,
+                          const int* rowptr, const int* colidx, int rd, int cd, int k, void* work,
+                          long long work_bytes, int* tp, int* xoff, int* exec, int* code32,
+                          unsigned* pos, long long* scalars, void* stream) {
+  OP2HIP_REQUIRE(nelems >= 0 && nnodes >= 0 && rmap && cmap && rowptr && colidx && tp && xoff &&
+                     exec && code32 && pos && scalars && work && k > 0,
+                 "bad row-block plan arguments");
+  OP2HIP_REQUIRE(e0 > 0 && e0 <= rarity && e0 <= kMaxCols && e1 > 0 && e1 <= carity && rd > 0 &&
+                     cd > 0,
+                 "bad row-block plan extents");
+  const long long ninc = static_cast<long long>(nelems) * e0;
+  OP2HIP_REQUIRE(ninc <= 0x7fffffffLL, "row-block plan of %lld records, beyond int32", ninc);
+  const int nblk = static_cast<int>((static_cast<long long>(nnodes) + k - 1) / k);
+  RowsWork w = rows_layout(nnodes, ninc, nblk);
+  OP2HIP_REQUIRE(work_bytes >= static_cast<long long>(w.bytes),
+                 "row-block plan workspace of %lld bytes, %zu needed", work_bytes, w.bytes);
+  const int W = (e1 + 3) / 4;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  char* base = static_cast<char*>(work);
+  int* cnt = at<int>(base, w.cnt);
+  u64* key = at<u64>(base, w.key);
+  int* longlist = at<int>(base, w.longlist);
+  int* nlong = at<int>(base, w.nlong);
+  long long* sums = at<long long>(base, w.sums);
+  long long* total = at<long long>(base, w.total);
+  int* cntb = at<int>(base, w.cntb);
+  u64* xkey = at<u64>(base, w.xkey);
+  int* scal = at<int>(base, w.scal);
+  Cols C{};
+  C.n = e0;
+  for (int c = 0; c < e0; ++c) {
+    C.v[c] = rmap;
+    C.dim[c] = rarity;
+    C.col[c] = c;
+  }
+  hipLaunchKernelGGL(zero_ints, dim3((nnodes + 1 + 255) / 256), dim3(256), 0, s, cnt,
+                     static_cast<long long>(nnodes) + 1);
+  hipLaunchKernelGGL(zero_ints, dim3((nblk + 1 + 255) / 256), dim3(256), 0, s, cntb,
+                     static_cast<long long>(nblk) + 1);
+  hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, nlong, 1LL);
+  hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, scal, 8LL);
+  // records per node, in ascending (rank, i0)
+  if (ninc > 0)
+    hipLaunchKernelGGL(count_targets, dim3((ninc + 255) / 256), dim3(256), 0, s, C, nelems,
+                       nnodes, cnt);
+  exclusive_scan(cnt, nnodes, tp, cnt, sums, total, s);
+  if (ninc > 0)
+    hipLaunchKernelGGL(scatter_keys, dim3((ninc + 255) / 256), dim3(256), 0, s, C, nelems,
+                       nnodes, rank, cnt, key);
+  if (nnodes > 0) {
+    long long threads = static_cast<long long>(nnodes) * 64;
+    hipLaunchKernelGGL(sort_targets_wave, dim3((threads + 255) / 256), dim3(256), 0, s, tp,
+                       nnodes, key, longlist, nlong);
+    hipLaunchKernelGGL(sort_targets_long, dim3(kLongGrid), dim3(kThreads), 0, s, tp, key,
+                       longlist, nlong);
+  }
+  // exec lists per block, in ascending rank
+  const dim3 egrid((nelems + 255) / 256 > 0 ? (nelems + 255) / 256 : 1);
+  if (nelems > 0)
+    hipLaunchKernelGGL(rows_exec_entries<true>, egrid, dim3(256), 0, s, rmap, rarity, e0, nelems,
+                       nnodes, k, rank, cntb, xkey);
+  exclusive_scan(cntb, nblk, xoff, cntb, sums, total, s);
+  if (nelems > 0)
+    hipLaunchKernelGGL(rows_exec_entries<false>, egrid, dim3(256), 0, s, rmap, rarity, e0, nelems,
+                       nnodes, k, rank, cntb, xkey);
+  if (nblk > 0) {
+    const int grid = nblk < 65536 ? nblk : 65536;
+    hipLaunchKernelGGL(rows_sort_exec, dim3(grid), dim3(kThreads), 0, s, xoff, nblk, xkey, exec);
+    hipLaunchKernelGGL(rows_stats, dim3((nblk + 255) / 256), dim3(256), 0, s, rowptr, nnodes, rd,
+                       k, nblk, xoff, scal);
+  }
+  if (nnodes > 0)
+    hipLaunchKernelGGL(rows_records, dim3((nnodes + 255) / 256), dim3(256), 0, s, key, tp, nnodes,
+                       e0, k, xkey, xoff, cmap, carity, e1, rowptr, colidx, rd, cd, W, code32, pos,
+                       scal);
+  int host[8];
+  int hsum = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(host, scal, sizeof(host), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hsum, xoff + nblk, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    op2hip::set_error("row-block plan build failed: %s", hipGetErrorString(e));
+    return OP2HIP_ERR_HIP;
+  }
+  scalars[0] = host[0];
+  scalars[1] = host[1];
+  scalars[2] = host[2];
+  scalars[3] = hsum;
+  scalars[4] = host[4];
+  return OP2HIP_OK;
+}
+
+// The codes of a row-block plan narrowed to 16 bits (every code < 65536).
+int op2hip_mat_rows_pack16(const int* code32, long long n, unsigned short* out, void* stream) {
+  OP2HIP_REQUIRE(n >= 0 && (n == 0 || (code32 && out)), "bad pack arguments");
+  if (n > 0)
+    hipLaunchKernelGGL(rows_pack16, dim3((n + 255) / 256), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), code32, n, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    op2hip::set_error("pack failed: %s", hipGetErrorString(e));
+    return OP2HIP_ERR_HIP;
+  }
+  return OP2HIP_OK;
+}
+
+}  // extern "C"

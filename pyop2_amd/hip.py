@@ -95,6 +95,9 @@ def _h2d(dev, host):
     """Copy a host numpy array's bytes into a device byte tensor."""
     if host.nbytes == 0:
         return
+    if torch.cuda.is_current_stream_capturing():
+        raise RuntimeError("host data reached the device during graph capture: run the "
+                           "captured function eagerly once before capturing it")
     src = torch.from_numpy(np.ascontiguousarray(host).reshape(-1).view(np.uint8))
     dev[:host.nbytes].copy_(src)
 
@@ -908,6 +911,142 @@ class DeviceOwnerPlan(object):
         return OwnerPlan.block_elements(self, b)
 
 
+class RowsMatPlan(object):
+    """Row-block plan of the owner Mat "rows" template, built on the device
+    (csrc/plan_device.hip op2hip_mat_rows_build).  Block b owns the row
+    nodes [b*k, min(nnodes, (b+1)*k)) of the device layout; its elements
+    (``exec[xoff[b]:xoff[b+1]]``, device numbering) are those naming one of
+    its nodes as a row, in ascending user order (``rank``); per row node,
+    ``tp`` delimits its records: ``code`` = slot * e0 + i0 (uint16 when
+    ``c16``) in ascending (user element, row position) order and ``pos``,
+    ceil(e1 / 4) words of column positions (byte j: the position of the
+    element's column node j among the row's node columns).  ``fits`` is
+    False when a row has more than 256 node columns (the positions would not
+    fit a byte); such a plan is not used."""
+
+    def __init__(self, n, arg, e0, e1, k, rank=None):
+        rm, cm = arg.map[0], arg.map[1]
+        sp = arg.data.sparsity
+        rd, cd = arg.data.dims
+        nnodes = rm.dataset.total_size
+        self.n, self.k, self.nnodes, self.e0, self.e1 = n, k, nnodes, e0, e1
+        self.nblk = (nnodes + k - 1) // k
+        ninc = n * e0
+        W = (e1 + 3) // 4
+        wbytes = nat.lib.op2hip_mat_rows_workspace(n, nnodes, e0, k)
+        if wbytes < 0:
+            raise RuntimeError("bad row-block plan arguments")
+        work = _dev_bytes(wbytes)
+        i32 = lambda m: torch.empty(max(m, 1), dtype=torch.int32, device=device())
+        # padding the kernel may load past the ends (never used): exec
+        # ROWS_EXEC_PAD zeros (element 0), records ROWS_RECORDS_PAD entries
+        d = {'tp': i32(nnodes + 1), 'xoff': i32(self.nblk + 1),
+             'exec': torch.zeros(ninc + ROWS_EXEC_PAD, dtype=torch.int32, device=device()),
+             'pos': torch.zeros((ninc + ROWS_RECORDS_PAD) * W, dtype=torch.int32, device=device())}
+        c32 = i32(ninc)
+        sc = (ctypes.c_longlong * 5)()
+        P = lambda t: ctypes.c_void_p(_ptr(t))
+        nat.check(nat.lib.op2hip_mat_rows_build(
+            n, nnodes, P(rm._device_values()), rm.dim, e0, P(cm._device_values()), cm.dim, e1,
+            P(rank), P(sp._dev_rowptr), P(sp._dev_colidx), rd, cd, k, P(work),
+            int(work.numel()), P(d['tp']), P(d['xoff']), P(d['exec']), P(c32), P(d['pos']), sc,
+            ctypes.c_void_p(stream())), 'row-block Mat plan')
+        del work
+        self.max_code, self.max_exec, self.max_slice, self.sum_exec, flags = [int(x) for x in sc]
+        if flags & 1:
+            raise RuntimeError("an element names a matrix entry outside the sparsity pattern")
+        if flags & 4:
+            raise RuntimeError("inconsistent row-block Mat plan")
+        # the template reads 16-bit codes: a block of more than 65536 / e0
+        # elements does not fit
+        self.fits = not flags & 2 and self.max_code <= 0xffff
+        self.c16 = True
+        d['code'] = torch.zeros(ninc + ROWS_RECORDS_PAD, dtype=torch.int16, device=device())
+        if self.fits and ninc:
+            nat.check(nat.lib.op2hip_mat_rows_pack16(P(c32), ninc, P(d['code']),
+                                                      ctypes.c_void_p(stream())), 'pack codes')
+        self._dev = d
+        self._xmaps = {}
+        self._sizes = {'tp': nnodes + 1, 'xoff': self.nblk + 1, 'exec': ninc, 'code': ninc,
+                       'pos': ninc * W}
+
+    @property
+    def redundancy(self):
+        """Element executions per element."""
+        return self.sum_exec / float(self.n) if self.n else 0.0
+
+    def device_arrays(self):
+        return self._dev
+
+    def xmap(self, m):
+        """The rows of map ``m`` (device values) in exec-list order, padded
+        like exec (element 0's row): what the template's element pipeline
+        reads instead of element numbers and the map (cached per map
+        array)."""
+        dv = m._device_values()
+        hit = self._xmaps.get(dv.data_ptr())
+        if hit is None:
+            idx = self._dev['exec'][:self.sum_exec + ROWS_EXEC_PAD].long()
+            hit = dv.view(torch.int32).view(-1, m.dim).index_select(0, idx).contiguous()
+            self._xmaps[dv.data_ptr()] = hit
+        return hit
+
+    def host(self, name):
+        """A host copy of one plan array (tests)."""
+        t = self._dev[name][:self._sizes[name]]
+        a = t.cpu().numpy()
+        if name == 'code' and self.c16:
+            a = a.view(np.uint16)
+        if name == 'pos':
+            a = a.view(np.uint32)
+        return a.astype(np.int64)
+
+
+_rows_cache = {}
+# largest local matrix (values per element) the rows template takes
+ROWS_MAX_ENTRIES = 16
+# padding entries of the row-block plan arrays (loads past the ends need no
+# guard): exec for three chunks of up to 1024 threads ahead, records for one
+# register batch
+ROWS_EXEC_PAD = 4 * 1024
+ROWS_RECORDS_PAD = 16
+# LDS a row block may use by default (a 64-thread block of 32 Laplace rows
+# takes ~12 KB)
+ROWS_LDS_LIMIT = 52 * 1024
+
+
+def get_rows_plan(iterset, n, args, bs, lds_of, lds_limit):
+    """Cached row-block plan of an owner Mat assembly, or None: ``k`` row
+    nodes per block (``hip_mat_rows_k``, default the block size), halved
+    until the block's LDS ``lds_of(k, largest slice)`` fits ``lds_limit``;
+    None when no k from 8 up fits or a row's column positions do not fit a
+    byte."""
+    a = [x for x in args if x._is_mat][0]
+    sp = a.data.sparsity
+    e0, e1 = a.map[0].dim, a.map[1].dim
+    k = min(int(cfg.get('hip_mat_rows_k') or bs), bs)
+    key = (id(iterset), n, id(a.map[0]), id(a.map[1]), id(sp), k, bs, lds_limit)
+    hit = _rows_cache.get(key)
+    if hit is not None and hit[0] is sp:
+        return hit[1]
+    lay = layout.of(iterset)
+    rank = None if lay.identity else _device_perm(lay)
+    plan = None
+    # the chunk's local matrices alone may not fit
+    while k >= 8 and lds_of(k, 0) <= lds_limit:
+        p = RowsMatPlan(n, a, e0, e1, k, rank)
+        if not p.fits:
+            break
+        if lds_of(k, p.max_slice) <= lds_limit:
+            plan = p
+            break
+        k //= 2
+    if len(_rows_cache) >= _CALL_CACHE_MAX:
+        _rows_cache.pop(next(iter(_rows_cache)))
+    _rows_cache[key] = (sp, plan)
+    return plan
+
+
 _owner_cache = {}
 OWNER_LDS_LIMIT = 64 * 1024
 # MI355X (gfx950) compute unit: 160 KB of LDS, 4 SIMDs of at most 8 waves
@@ -1125,7 +1264,7 @@ class _LaunchEntry(object):
         for a in args:
             if a._is_direct and a.data.dataset is not iterset:
                 layout.link(a.data.dataset, iterset)
-        if mode == 'owner':
+        if mode in ('owner', 'rows'):
             m0, i0 = _owner_columns(args)[0]
             layout.prefer(iterset, m0, max(i0, 0))
         layout.of(iterset)
@@ -1142,7 +1281,16 @@ class _LaunchEntry(object):
         _, src = pl._source(mode, bs, consts)
         plan = None
         lds = 0
-        if mode == 'owner':
+        if mode == 'rows':
+            limit = min(int(cfg.get('hip_mat_rows_lds') or ROWS_LDS_LIMIT),
+                        _state.get('lds_per_block') or 65536)
+            plan = get_rows_plan(iterset, n, args, bs, src.rows_lds_bytes, limit)
+            if plan is None:
+                raise _NoOwnerPlan()
+            nparts = plan.nblk
+            lds = src.rows_lds_bytes(plan.k, plan.max_slice)
+            grid_stream = 0
+        elif mode == 'owner':
             slot_bytes = src.owner_slot_bytes()
             lds_limit = None
             if any(a._is_mat for a in args):
@@ -1188,6 +1336,8 @@ class _LaunchEntry(object):
                 vals.append(_ptr(t))
             elif role == 'data':
                 vals.append(_ptr(a.data._dev))
+            elif role == 'rows_xmap':
+                vals.append(_ptr(plan.xmap(a.map)))
             elif role == 'map':
                 vals.append(_ptr(a.map._device_values()))
             elif role == 'mat_vals':
@@ -1210,6 +1360,17 @@ class _LaunchEntry(object):
             if mode == 'stream':
                 argv = [0 if r == 'start' else n if r == 'count' else r for r in vals]
                 launches.append((loop.fn, grid_stream, bs, _argv(argv), 0))
+            elif mode == 'rows':
+                d = plan.device_arrays()
+                self.keep.append(plan)
+                names = {'own_tp': 'tp', 'own_xoff': 'xoff', 'own_exec': 'exec',
+                         'own_contrib': 'code', 'own_pos': 'pos'}
+                scal = {'start': 0, 'count': plan.nblk, 'own_c16': int(plan.c16),
+                        'own_k': plan.k, 'own_nnodes': plan.nnodes, 'own_slice': plan.max_slice}
+                argv = [scal[r] if r in scal else _ptr(d[names[r]]) if r in names else r
+                        for r in vals]
+                if plan.nblk:
+                    launches.append((loop.fn, plan.nblk, bs, _argv(argv), lds))
             elif mode == 'owner':
                 pdev = plan.device_arrays()
                 self.keep.append(plan)
@@ -1450,6 +1611,20 @@ class ParLoop(base.ParLoop):
         else:
             obs = cfg.get('hip_owner_block_size') or bs
             fill = cfg.get('hip_owner_fill')
+        if (mode == 'owner' and any(a._is_mat for a in self.args) and
+                cfg.get('hip_mat_rows', True) and
+                not any(a._is_global_reduction for a in self.args)):
+            # row blocks first for small local matrices: each element run by
+            # fewer blocks, the CSR slice summed in LDS.  (Larger ones, e.g.
+            # the 2x2-block vector mass with 64 values, keep the owner
+            # template: their chunk of local matrices crowds the LDS.)
+            m = [a for a in self.args if a._is_mat][0]
+            ext = self._it_space.extents
+            if ext[0] * ext[1] * m.data.dims[0] * m.data.dims[1] <= ROWS_MAX_ENTRIES:
+                try:
+                    return _LaunchEntry(self, 'rows', cfg.get('hip_mat_rows_block_size') or bs)
+                except _NoOwnerPlan:
+                    pass
         try:
             with cfg.override(hip_owner_fill=fill):
                 return _LaunchEntry(self, mode, obs if mode == 'owner' else bs)
@@ -1517,7 +1692,8 @@ _VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill', 'hi
                  'hip_owner_targets', 'hip_owner', 'hip_partition_size', 'hip_reorder',
                  'hip_stage_reads', 'hip_stage_writes', 'hip_stream_unroll',
                  'hip_stream_blocks', 'hip_owner_prefetch', 'hip_owner_waves',
-                 'hip_owner_mat_lds', 'hip_colouring')
+                 'hip_owner_mat_lds', 'hip_colouring', 'hip_mat_rows', 'hip_mat_rows_block_size',
+                 'hip_mat_rows_k', 'hip_mat_rows_lds')
 
 
 def _written_buffers(pl):

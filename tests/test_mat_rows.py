@@ -1,0 +1,169 @@
+"""Owner Mat assembly by row blocks (the "rows" template).
+
+Plan: the row-block plan built on the device (op2hip_mat_rows_build) equals,
+array for array, a plain construction from its definition: per row node its
+records (element, row position) in ascending (user element, row position)
+order, coded slot * e0 + i0 with slot the element's place in its block's
+exec list; per block the elements naming one of its nodes in ascending user
+order; per record the position of each column node in the row.  Checked on
+scalar and 2x2-block patterns, identity and shuffled user orders, blocks
+of 8 to 64 nodes.
+
+Assembly: P1 mass and Laplace matrices on triangles and tetrahedra, a
+vector mass, a Mat WRITE and an element naming one node twice are
+bit-exact to the oracle's sequential loop (mat_utils.cxx:4-28 addto order)
+and to the owner template; the rows template is the one that ran."""
+
+import numpy as np
+import pytest
+
+from oracle import sequential as seq
+from pyop2_amd.workloads import fem, meshes
+
+pytestmark = pytest.mark.gpu
+
+
+def _expected(en_r, en_c, rank, nnodes, k, rp, ci, rd, cd):
+    n, e0 = en_r.shape
+    e1 = en_c.shape[1]
+    user = np.argsort(rank, kind='stable')
+    recs = [[] for _ in range(nnodes)]
+    for e in user:
+        for i0 in range(e0):
+            recs[en_r[e, i0]].append((e, i0))
+    nblk = (nnodes + k - 1) // k
+    exe = []
+    for b in range(nblk):
+        els = {e for t in range(b * k, min(nnodes, (b + 1) * k)) for e, _ in recs[t]}
+        exe.append(sorted(els, key=lambda e: rank[e]))
+    tp = np.zeros(nnodes + 1, dtype=np.int64)
+    tp[1:] = np.cumsum([len(r) for r in recs])
+    codes, pos = [], []
+    W = (e1 + 3) // 4
+    for t in range(nnodes):
+        slot = {e: s for s, e in enumerate(exe[t // k])}
+        cols = list(ci[rp[t * rd]:rp[t * rd + 1]:cd] // cd)
+        for e, i0 in recs[t]:
+            codes.append(slot[e] * e0 + i0)
+            words = [0] * W
+            for j in range(e1):
+                words[j // 4] |= cols.index(en_c[e, j]) << (8 * (j % 4))
+            pos += words
+    xoff = np.zeros(nblk + 1, dtype=np.int64)
+    xoff[1:] = np.cumsum([len(x) for x in exe])
+    return tp, xoff, [np.array(x, dtype=np.int64) for x in exe], np.array(codes), np.array(pos)
+
+
+@pytest.mark.parametrize('dim,k,shuffle', [(1, 8, False), (1, 13, True), (2, 29, True),
+                                           (1, 64, True)])
+def test_rows_plan_matches_definition(hip, dim, k, shuffle):
+    from pyop2_amd import hip as H, layout
+    mesh = meshes.TetMesh(4, seed=2)
+    en = mesh.elem_node.astype(np.int32)
+    n, nv = en.shape[0], mesh.nvert
+    el = hip.Set(n, 1)
+    nodes = hip.Set(nv, dim)
+    m = hip.Map(el, nodes, 4, en)
+    layout._group(el).layout = layout.IDENTITY
+    layout._group(nodes).layout = layout.IDENTITY
+    sp = hip.Sparsity((m, m))
+    A = hip.Mat(sp, np.float64)
+    rank = (np.random.default_rng(k).permutation(n) if shuffle else np.arange(n)).astype(np.int32)
+    arg = A((m[hip.i[0]], m[hip.i[1]]), hip.INC)
+    plan = H.RowsMatPlan(n, arg, 4, 4, k, H.torch.from_numpy(rank).to(H.device()))
+    rp = sp._dev_rowptr.cpu().numpy().astype(np.int64)
+    ci = sp._dev_colidx[:sp.nz].cpu().numpy().astype(np.int64)
+    tp, xoff, exe, codes, pos = _expected(en, en, rank, nv, k, rp, ci, dim, dim)
+    assert plan.fits and plan.nblk == len(exe)
+    assert np.array_equal(plan.host('tp'), tp)
+    assert np.array_equal(plan.host('xoff'), xoff)
+    ex = plan.host('exec')
+    for b in range(plan.nblk):
+        assert np.array_equal(ex[xoff[b]:xoff[b + 1]], exe[b])
+    assert np.array_equal(plan.host('code'), codes)
+    assert np.array_equal(plan.host('pos'), pos)
+    assert plan.max_code == codes.max()
+    assert plan.max_exec == max(len(x) for x in exe)
+    assert plan.sum_exec == xoff[-1]
+    slices = [rp[min(nv, (b + 1) * k) * dim] - rp[b * k * dim] for b in range(plan.nblk)]
+    assert plan.max_slice == max(slices)
+
+
+def _mass_tri(W, mesh, mode, kernel=fem.MASS_TRI, name='mass_tri'):
+    nodes = W.Set(mesh.nvert)
+    vn = W.Set(mesh.nvert, 2)
+    el = W.Set(mesh.nelem)
+    en = W.Map(el, nodes, 3, mesh.elem_node)
+    ev = W.Map(el, vn, 3, mesh.elem_node)
+    x = W.Dat(vn, mesh.coords.copy(), np.float64)
+    A = W.Mat(W.Sparsity((en, en)), np.float64)
+    W.par_loop(W.Kernel(kernel, name), el(3, 3), A((en[W.i[0]], en[W.i[1]]), mode),
+               x(ev, W.READ))
+    return np.array(A.values)
+
+
+def _loop_modes(hip):
+    from pyop2_amd import hip as H
+    return {e.loop.source.mode for e in H._call_cache.values() if e.loop is not None}
+
+
+@pytest.mark.parametrize('opts', [{}, {'hip_mat_rows_k': 16}, {'hip_mat_rows_block_size': 128},
+                                  {'hip_mat_rows_k': 8, 'hip_mat_rows_block_size': 64}])
+def test_rows_mass_tri_bitexact(hip, opts):
+    from pyop2_amd import configuration as cfg, hip as H
+    mesh = meshes.TriMesh(40, seed=7)
+    ref = _mass_tri(seq, mesh, seq.INC)
+    H._call_cache.clear()
+    with cfg.override(**opts):
+        got = _mass_tri(hip, mesh, hip.INC)
+        assert 'rows' in _loop_modes(hip)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_rows_laplace_and_vector_mass_bitexact(hip):
+    """Tetrahedra: the Laplace matrix through the rows template, bit-exact
+    to the oracle and to the owner template; the vector mass (whose local
+    matrices do not fit the rows template's LDS) bit-exact too."""
+    from pyop2_amd import configuration as cfg
+    mesh = meshes.TetMesh(9, seed=3)
+    kappa = np.random.default_rng(4).lognormal(0, 1, mesh.nvert)
+    o = fem.Assembly3D(seq, mesh, kappa, name='o3r')
+    o.laplace()
+    o.vector_mass()
+    out = {}
+    for rows in (True, False):
+        with cfg.override(hip_mat_rows=rows):
+            a = fem.Assembly3D(hip, mesh, kappa, name='h3r%d' % rows)
+            a.laplace()
+            a.vector_mass()
+            out[rows] = (np.array(a.A.array), np.array(a.M.array))
+    for rows in (True, False):
+        np.testing.assert_array_equal(out[rows][0], o.A.array)
+        np.testing.assert_array_equal(out[rows][1], o.M.array)
+
+
+def test_rows_insert_and_repeated_node(hip):
+    """Mat WRITE keeps the last element's value (sequential order), and an
+    element naming one node at two positions adds both terms in order."""
+    mesh = meshes.TriMesh(11, seed=9)
+    code = """void k(double A[1][1], double *x[2], int i, int j) {
+      A[0][0] = x[i][0] * 10.0 + x[j][1] + (double)(i * 3 + j); }"""
+    ref = _mass_tri(seq, mesh, seq.WRITE, code, 'k')
+    np.testing.assert_array_equal(_mass_tri(hip, mesh, hip.WRITE, code, 'k'), ref)
+    en = mesh.elem_node.copy()
+    en[::7, 2] = en[::7, 0]
+    inc = """void k2(double A[1][1], double *x[2], int i, int j) {
+      A[0][0] += x[i][0] * 0.3 + x[j][1] * 0.7 + 0.1 * (double)(i + 2 * j); }"""
+    out = []
+    for W in (seq, hip):
+        nodes = W.Set(mesh.nvert)
+        vn = W.Set(mesh.nvert, 2)
+        el = W.Set(mesh.nelem)
+        m = W.Map(el, nodes, 3, en)
+        mv = W.Map(el, vn, 3, en)
+        x = W.Dat(vn, mesh.coords.copy(), np.float64)
+        A = W.Mat(W.Sparsity((m, m)), np.float64)
+        W.par_loop(W.Kernel(inc, 'k2'), el(3, 3), A((m[W.i[0]], m[W.i[1]]), W.INC),
+                   x(mv, W.READ))
+        out.append(np.array(A.values))
+    np.testing.assert_array_equal(out[1], out[0])
