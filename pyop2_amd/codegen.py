@@ -1031,9 +1031,6 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   %(t)s* s_acc = (%(t)s*)op2_lds;
   %(t)s* s_val = s_acc + (op2_i64)own_slice;
   int* s_rp = (int*)(s_val + %(pitch)d * %(bs)d);
-  for (int i = threadIdx.x; i < nsl; i += %(bs)d) s_acc[i] = d%(p)d[(op2_i64)base + i];
-  for (int i = threadIdx.x; i <= nr * %(rd)d; i += %(bs)d)
-    s_rp[i] = rp%(p)d[(op2_i64)t0 * %(rd)d + i] - base;
   // a row thread's records: [rec, rend)
   int rec = 0, rend = 0;
   if ((int)threadIdx.x < nr) {
@@ -1149,15 +1146,57 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                     '    }',
                     '    %s = p_exec[ex0 + c0 + 2 * %d + threadIdx.x];' % (ev, bs),
                     rows, ind(fast, 10), rows2, ind(slow, 8), rows3, '    }']
-        out = [head, ind(decls, 2), ind(decls_n, 2), ind(ml('ex0 + threadIdx.x'), 2),
-               ind(loads, 2), ind(ml('ex0 + %d + threadIdx.x' % bs), 2),
+        slice_pre = """  // the first %(U)d * %(bs)d entries of the block's CSR slice: their loads go
+  // out first (indices past the end read the last entry again) and their
+  // LDS stores wait until the first element loads are issued
+  %(t)s sv[%(U)d];
+  if (nsl > 0) {
+#pragma unroll
+    for (int u = 0; u < %(U)d; ++u) {
+      const int i = u * %(bs)d + (int)threadIdx.x;
+      sv[u] = d%(p)d[(op2_i64)base + (i < nsl ? i : nsl - 1)];
+    }
+  }""" % dict(sub, U=8)
+        slice_in = """#pragma unroll
+  for (int u = 0; u < %(U)d; ++u) {
+    const int i = u * %(bs)d + (int)threadIdx.x;
+    if (i < nsl) s_acc[i] = sv[u];
+  }
+  // the rest of the slice, %(U)d loads per thread before their stores
+  for (int i0 = %(U)d * %(bs)d; i0 < nsl; i0 += %(U)d * %(bs)d) {
+#pragma unroll
+    for (int u = 0; u < %(U)d; ++u) {
+      const int i = i0 + u * %(bs)d + (int)threadIdx.x;
+      sv[u] = d%(p)d[(op2_i64)base + (i < nsl ? i : nsl - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < %(U)d; ++u) {
+      const int i = i0 + u * %(bs)d + (int)threadIdx.x;
+      if (i < nsl) s_acc[i] = sv[u];
+    }
+  }
+  for (int i = threadIdx.x; i <= nr * %(rd)d; i += %(bs)d)
+    s_rp[i] = rp%(p)d[(op2_i64)t0 * %(rd)d + i] - base;""" % dict(sub, U=8)
+        slice_out = """  for (int i0 = 0; i0 < nsl; i0 += %(U)d * %(bs)d) {
+    %(t)s sv[%(U)d];
+#pragma unroll
+    for (int u = 0; u < %(U)d; ++u) {
+      const int i = i0 + u * %(bs)d + (int)threadIdx.x;
+      sv[u] = s_acc[i < nsl ? i : nsl - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < %(U)d; ++u) {
+      const int i = i0 + u * %(bs)d + (int)threadIdx.x;
+      if (i < nsl) d%(p)d[(op2_i64)base + i] = sv[u];
+    }
+  }""" % dict(sub, U=8)
+        out = [head, slice_pre, ind(decls, 2), ind(decls_n, 2), ind(ml('ex0 + threadIdx.x'), 2),
+               ind(loads, 2), ind(ml('ex0 + %d + threadIdx.x' % bs), 2), slice_in,
                '  for (int c0 = 0; c0 < nex; c0 += %(bs)d) {' % sub]
         out += step('e', loads_n, compute, 'lm%d' % p)
         out += ['    c0 += %(bs)d;' % sub, '    if (c0 >= nex) break;']
         out += step('e_n', loads, compute_n, 'lm%d_n' % p)
-        out += ['  }',
-                '  for (int i = threadIdx.x; i < nsl; i += %(bs)d) d%(p)d[(op2_i64)base + i] = s_acc[i];'
-                % sub]
+        out += ['  }', slice_out]
         return out
 
     def rows_pitch(self):

@@ -978,6 +978,14 @@ class RowsMatPlan(object):
     def device_arrays(self):
         return self._dev
 
+    def stage(self, m):
+        """Per-block staging lists through map ``m`` (each block's distinct
+        targets and each slot's positions in them, the reference's ind_map
+        and loc_map) are switched off: their device build faulted on its
+        first GPU runs and the cause is not established.  The rows template
+        gathers element rows from the Dats directly."""
+        raise RuntimeError("row-block staging lists are switched off")
+
     def xmap(self, m):
         """The rows of map ``m`` (device values) in exec-list order, padded
         like exec (element 0's row): what the template's element pipeline

@@ -89,6 +89,44 @@ def test_rows_plan_matches_definition(hip, dim, k, shuffle):
     assert plan.max_slice == max(slices)
 
 
+@pytest.mark.skip(reason="staging lists switched off: their device build faulted with a memory "
+                         "aperture violation on its first GPU runs; cause not established")
+@pytest.mark.parametrize('k', [8, 29])
+def test_rows_staging_lists(hip, k):
+    """Per block the sorted distinct targets its elements name through a
+    map (the reference's ind_map) and per exec slot their positions in that
+    list (its loc_map)."""
+    from pyop2_amd import hip as H, layout
+    mesh = meshes.TetMesh(4, seed=5)
+    en = mesh.elem_node.astype(np.int32)
+    n, nv = en.shape[0], mesh.nvert
+    el = hip.Set(n, 1)
+    nodes = hip.Set(nv, 1)
+    m = hip.Map(el, nodes, 4, en)
+    layout._group(el).layout = layout.IDENTITY
+    layout._group(nodes).layout = layout.IDENTITY
+    A = hip.Mat(hip.Sparsity((m, m)), np.float64)
+    rank = np.random.default_rng(k).permutation(n).astype(np.int32)
+    plan = H.RowsMatPlan(n, A((m[hip.i[0]], m[hip.i[1]]), hip.INC), 4, 4, k,
+                         H.torch.from_numpy(rank).to(H.device()))
+    sn, sc, xl, smax = plan.stage(m)
+    sn, sc = sn.cpu().numpy(), sc.cpu().numpy()
+    xl = xl.cpu().numpy().view(np.uint16)
+    xoff, ex = plan.host('xoff'), plan.host('exec')
+    longest = 0
+    for b in range(plan.nblk):
+        els = ex[xoff[b]:xoff[b + 1]]
+        want = np.unique(en[els].ravel())
+        got = sn[xoff[b] * 4:xoff[b] * 4 + sc[b]]
+        assert np.array_equal(got, want)
+        loc = xl[xoff[b] * 4:xoff[b + 1] * 4].reshape(-1, 4)
+        assert np.array_equal(want[loc], en[els])
+        longest = max(longest, len(want))
+    assert smax == max(2, (longest + 1) & ~1)
+    # the padding past the last slot names position 0
+    assert (xl[xoff[-1] * 4:] == 0).all()
+
+
 def _mass_tri(W, mesh, mode, kernel=fem.MASS_TRI, name='mass_tri'):
     nodes = W.Set(mesh.nvert)
     vn = W.Set(mesh.nvert, 2)
