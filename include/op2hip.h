@@ -58,6 +58,13 @@ int op2hip_module_function(void* mod, const char* name, void** fn_out);
 int op2hip_module_global(void* mod, const char* name, void** dptr, size_t* bytes);
 /* Copy `bytes` host bytes to a module global on `stream`, then wait. */
 int op2hip_set_global(void* dptr, const void* src, size_t bytes, void* stream);
+/* Store up to 256 host bytes at `dptr` in `stream` order without waiting
+ * (a kernel carrying them as its argument; capturable in a graph).  A
+ * Global's value reaching the device: replaces the reduction-buffer
+ * upload of pyop2/cuda.py:327-339 (gpuarray.to_gpu / fill), which the
+ * reference follows with a stream synchronize per reduction
+ * (pyop2/cuda.py:353-356). */
+int op2hip_store_small(void* dptr, const void* src, size_t bytes, void* stream);
 /* Compile (hiprtc, cached), load and resolve `entry` in one call; the
  * module belongs to the caller.  The JITModule compile+lookup of
  * pyop2/cuda.py:602-651 as one entry point. */

@@ -67,6 +67,9 @@ class Arg(object):
 
     User code builds these by calling a Dat, Global or Mat."""
 
+    # a backend's key of this argument (it never changes once made)
+    _fkey = None
+
     def __init__(self, data=None, map=None, idx=None, access=None):
         self._dat = data
         self._map = map
@@ -442,8 +445,22 @@ class Dat(DataCarrier):
             self._id = uid
         self._name = name or "dat_%d" % self._id
 
+    _argmemo = None
+
     def __call__(self, path, access):
-        return self._arg(canonical(path), canonical(access))
+        # a time loop builds the same arguments every step: one Arg per
+        # (Map or Map index, access), kept with the objects it was made from
+        memo = self._argmemo
+        if memo is not None:
+            hit = memo.get((id(path), id(access)))
+            if hit is not None and hit[0] is path and hit[1] is access:
+                return hit[2]
+        a = self._arg(canonical(path), canonical(access))
+        if isinstance(path, Map) or getattr(path, '_template', False):
+            if memo is None:
+                memo = self._argmemo = {}
+            memo[(id(path), id(access))] = (path, access, a)
+        return a
 
     @validate_in(('access', _modes, ModeValueError))
     def _arg(self, path, access):
@@ -451,10 +468,7 @@ class Dat(DataCarrier):
             if path._dataset != self._dataset and path != IdentityMap:
                 raise MapValueError("Dataset of Map does not match Dataset of Dat.")
             return _new_arg(self, data=self, map=path, access=access)
-        else:
-            path._dat = self
-            path._access = access
-            return path
+        return _new_arg(self, data=self, map=path._map, idx=path._idx, access=access)
 
     @property
     def dataset(self):
@@ -781,7 +795,27 @@ class Map(object):
         self._uid = Map._globalcount
         Map._globalcount += 1
 
+    _imemo = None
+
     def __getitem__(self, index):
+        # one (Map, index) template per int or op2.i[n] index, so that the
+        # Dat call it is passed to can reuse the Arg it made from it
+        memo = self._imemo
+        if memo is not None and type(index) is int:
+            a = memo.get(index)
+            if a is not None:
+                return a
+        t = type(index)
+        key = index if t is int else ('i', index._index) if t is IterationIndex else None
+        if key is not None:
+            if memo is None:
+                memo = self._imemo = {}
+            a = memo.get(key)
+            if a is None:
+                a = self._index(index)
+                a._template = True
+                memo[key] = a
+            return a
         if isinstance(index, slice):
             raise NotImplementedError("Slicing maps is not allowed")
         return self._index(canonical(index))

@@ -1319,8 +1319,10 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         wavefront shuffle tree and the 16 wavefront sums in order."""
         nt, unroll = 1024, 4
         decl = ['op2_i64 nparts']
+        # the Globals are not __restrict__: one Global may be reduced into at
+        # two positions, each section then reads the one before it wrote
         for a in self.reductions:
-            decl += ['%s* __restrict__ g%d' % (a.ctype, a.pos),
+            decl += ['%s* g%d' % (a.ctype, a.pos),
                      'const %s* __restrict__ r%d' % (a.ctype, a.pos)]
         lines = ['extern "C" __global__ void __launch_bounds__(%d) %s(%s) {'
                  % (nt, self.reduce_entry, ', '.join(decl))]

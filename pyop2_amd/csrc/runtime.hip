@@ -244,6 +244,39 @@ int op2hip_set_global(void* dptr, const void* src, size_t bytes, void* stream) {
   return OP2HIP_OK;
 }
 
+namespace {
+constexpr int kSmallBytes = 256;
+struct SmallBytes {
+  unsigned char b[kSmallBytes];
+};
+
+// Lane i stores byte i: the bytes travel as the kernel's argument.
+__global__ void __launch_bounds__(kSmallBytes) store_small_bytes(unsigned char* __restrict__ dst,
+                                                                  SmallBytes v, int n) {
+  const int i = static_cast<int>(threadIdx.x);
+  if (i < n) dst[i] = v.b[i];
+}
+}  // namespace
+
+// Store up to 256 host bytes at `dptr` in `stream` order, without waiting:
+// the bytes are copied into a kernel argument before this returns, so the
+// host buffer may be reused at once and the store may be captured in a
+// graph.  A Global's value reaches the device this way.
+int op2hip_store_small(void* dptr, const void* src, size_t bytes, void* stream) {
+  OP2HIP_REQUIRE(dptr && src, "null argument to op2hip_store_small");
+  OP2HIP_REQUIRE(bytes <= static_cast<size_t>(kSmallBytes),
+                 "op2hip_store_small takes at most %d bytes (got %zu)", kSmallBytes, bytes);
+  if (bytes == 0) return OP2HIP_OK;
+  SmallBytes v;
+  std::memset(v.b, 0, sizeof(v.b));
+  std::memcpy(v.b, src, bytes);
+  hipLaunchKernelGGL(store_small_bytes, dim3(1), dim3(kSmallBytes), 0,
+                     static_cast<hipStream_t>(stream), static_cast<unsigned char*>(dptr), v,
+                     static_cast<int>(bytes));
+  OP2HIP_CHECK(hipGetLastError());
+  return OP2HIP_OK;
+}
+
 // Compile, load and look up in one call: `src` through hiprtc (with the
 // code-object cache), its module loaded, `entry` resolved.  The module is
 // the caller's (op2hip_module_destroy).

@@ -235,6 +235,16 @@ class Global(_DeviceData, base.Global):
         base.Global.__init__(self, *args, **kwargs)
         self._dev_init()
 
+    def _upload(self):
+        # a value of up to 256 bytes reaches the device in stream order, as
+        # a kernel argument: creating a Global in a loop costs no host wait
+        a = np.ascontiguousarray(self._data)
+        if a.nbytes > 256:
+            return _DeviceData._upload(self)
+        nat.check(nat.lib.op2hip_store_small(ctypes.c_void_p(_ptr(self._dev)),
+                                             a.ctypes.data_as(ctypes.c_void_p), a.nbytes,
+                                             ctypes.c_void_p(stream())), 'Global upload')
+
 
 class Const(base.Const):
     pass
@@ -1260,6 +1270,45 @@ class _LaunchEntry(object):
     __slots__ = ('loop', 'launches', 'to_device', 'written', 'dats', 'globals_red', 'mats',
                  'consts', 'keep')
 
+    def rebindable(self, args):
+        """A copy whose Global arguments can be exchanged for others of
+        the same shape (``rebind``): its own argument arrays, and per Global
+        argument position the (launch, slot) entries holding its device
+        pointer.  Reference-style drivers build a new Global per call
+        (demo/airfoil.py:124-130: ``rms = op2.Global(...)`` in the loop)."""
+        c = object.__new__(_LaunchEntry)
+        for k in ('loop', 'consts', 'dats', 'mats', 'globals_red'):
+            setattr(c, k, getattr(self, k, None))
+        c.launches = [(fn, grid, bs, ((ctypes.c_uint64 * n)(*arr), n), lds)
+                      for fn, grid, bs, (arr, n), lds in self.launches]
+        c.to_device = list(self.to_device)
+        c.written = list(self.written)
+        c.keep = list(self.keep)
+        slots = {}
+        for pos, a in enumerate(args):
+            if a._is_global:
+                ptr = _ptr(a.data._dev)
+                slots[pos] = (a.data, [(li, ai) for li, (_, _, _, (arr, n), _) in
+                                       enumerate(c.launches) for ai in range(n)
+                                       if arr[ai] == ptr])
+        c.globals_red = slots
+        return c
+
+    def rebind(self, args):
+        """Point the Global arguments at ``args``' Globals."""
+        for pos, (cur, where) in list(self.globals_red.items()):
+            new = args[pos].data
+            if new is cur:
+                continue
+            new._to_device()
+            ptr = _ptr(new._dev)
+            for li, ai in where:
+                self.launches[li][3][0][ai] = ptr
+            self.to_device = [new if d is cur else d for d in self.to_device]
+            self.written = [new if d is cur else d for d in self.written]
+            # the bound Globals stay alive through globals_red / to_device
+            self.globals_red[pos] = (new, where)
+
     def __init__(self, pl, mode, bs):
         consts = base.Const._definitions()
         self.consts = consts
@@ -1267,6 +1316,7 @@ class _LaunchEntry(object):
         n = pl._it_space.exec_size if pl.needs_exec_halo else pl._it_space.size
         iterset = pl._it_space.iterset
         self.keep = [pl.kernel, args, iterset]
+        self.globals_red = None
         # direct Dats on another Set of the same size share the iteration
         # set's layout (element e is one position in both)
         for a in args:
@@ -1479,6 +1529,14 @@ _CALL_CACHE_MAX = 512
 
 
 def _arg_key(a):
+    k = a._fkey
+    if k is None:
+        # an Arg never changes once made: its key is computed once
+        k = a._fkey = _arg_key_of(a)
+    return k
+
+
+def _arg_key_of(a):
     m = a.map
     if isinstance(m, (list, tuple)):
         mk = tuple(id(x) for x in m)
@@ -1770,7 +1828,19 @@ def _fast_key(kernel, it_space, args):
         head = (id(kernel), id(it_space.iterset), it_space.extents)
     else:
         head = (id(kernel), id(it_space), ())
-    return head + tuple(_arg_key(a) for a in args)
+    ks = []
+    for a in args:
+        k = a._fkey
+        if k is None:
+            if a._is_global:
+                # Globals by shape and by which positions share one Global:
+                # a cached call rebinds them
+                k = ('gbl', a.data.cdim, a.data.dtype.char, id(a.access),
+                     next(j for j, b in enumerate(args) if b.data is a.data))
+            else:
+                k = a._fkey = _arg_key_of(a)
+        ks.append(k)
+    return head + tuple(ks)
 
 
 def par_loop(kernel, it_space, *args):
@@ -1782,7 +1852,10 @@ def par_loop(kernel, it_space, *args):
     state = (cfg.version(), base.Const._defs_version)
     hit = _fast_cache.get(key) if key is not None else None
     if hit is not None and hit[0] == state:
-        hit[1].run()
+        ent = hit[1]
+        if ent.globals_red:
+            ent.rebind(args)
+        ent.run()
         for d in hit[2]:
             d.needs_halo_update = True
         return
@@ -1791,7 +1864,10 @@ def par_loop(kernel, it_space, *args):
     if key is not None and not cfg.get('hip_autotune'):
         if len(_fast_cache) >= _CALL_CACHE_MAX:
             _fast_cache.pop(next(iter(_fast_cache)))
-        _fast_cache[key] = (state, pl._entry_cached(),
+        ent = pl._entry_cached()
+        if any(a._is_global for a in pl.args):
+            ent = ent.rebindable(pl.args)
+        _fast_cache[key] = (state, ent,
                             [a.data for a in pl.args
                              if a._is_dat and a.access in (INC, WRITE, RW)])
 

@@ -314,12 +314,12 @@ class Airfoil(object):
                    self.p_q(self.pbevcell[0], W.READ), self.p_adt(self.pbecell[0], W.READ),
                    self.p_res(self.pbevcell[0], W.INC), self.p_bound(W.IdentityMap, W.READ))
 
-    def update(self):
+    def update(self, rms=None):
         W = self.W
         W.par_loop(self.k_update, self.cells,
                    self.p_qold(W.IdentityMap, W.READ), self.p_q(W.IdentityMap, W.WRITE),
                    self.p_res(W.IdentityMap, W.RW), self.p_adt(W.IdentityMap, W.READ),
-                   self.rms(W.INC))
+                   (self.rms if rms is None else rms)(W.INC))
 
     def iteration(self, inner=2):
         """One time step: save_soln, then ``inner`` x (adt, res, bres, update)."""
@@ -329,6 +329,21 @@ class Airfoil(object):
             self.res_calc()
             self.bres_calc()
             self.update()
+
+    def demo_iteration(self, inner=2):
+        """One time step as the reference's demo drives it
+        (demo/airfoil.py:89-133): a new ``rms`` Global for each update, and
+        the last one read back on the host at the end of the step.  Returns
+        sqrt(rms / cells)."""
+        W = self.W
+        self.save_soln()
+        for _ in range(inner):
+            self.adt_calc()
+            self.res_calc()
+            self.bres_calc()
+            rms = W.Global(1, 0.0, np.float64, 'rms')
+            self.update(rms)
+        return float(np.sqrt(rms.data[0] / self.cells.size))
 
 
 class AirfoilVector(Airfoil):

@@ -150,3 +150,38 @@ def test_1d_multi_inc_diff_global(backend, s1, d1):
     op2.par_loop(_k(INC1), s1, d1(op2.IdentityMap, op2.READ), g2(op2.INC))
     assert g2.data[0] == d1.data_ro.sum() + 10
     assert g.data[0] == d1.data_ro.sum()
+
+
+def test_new_global_each_call(backend, s1, d1):
+    """A driver building a fresh Global per call (demo/airfoil.py:124-130
+    makes ``rms`` inside its loop): each call reduces into its own Global
+    and reads its own READ Global; a repeated call may reuse the planned
+    launch, but never a previous call's Global."""
+    k = op2.Kernel("void kg(unsigned int *x, unsigned int *a, unsigned int *g) "
+                   "{ *g += *x * *a; }", 'kg')
+    total = int(d1.data_ro.sum())
+    for it in range(5):
+        a = op2.Global(1, it + 1, dtype=np.uint32)
+        g = op2.Global(1, 7 * it, dtype=np.uint32)
+        op2.par_loop(k, s1, d1(op2.IdentityMap, op2.READ), a(op2.READ), g(op2.INC))
+        assert g.data[0] == (total * (it + 1) + 7 * it) % 2 ** 32
+        assert a.data[0] == it + 1
+    if backend == 'hip':
+        from pyop2_amd import hip as H
+        assert any(e[1].globals_red for e in H._fast_cache.values())
+
+
+def test_new_global_aliasing(backend, s1, d1):
+    """One Global at two positions, then two Globals there: the second
+    call is not taken for the first one's repeat."""
+    k = op2.Kernel("void kh(unsigned int *x, unsigned int *g, unsigned int *h) "
+                   "{ *g += *x; *h += 1; }", 'kh')
+    total = int(d1.data_ro.sum())
+    for _ in range(2):
+        g = op2.Global(1, 0, dtype=np.uint32)
+        op2.par_loop(k, s1, d1(op2.IdentityMap, op2.READ), g(op2.INC), g(op2.INC))
+        assert g.data[0] == total + NELEMS
+        g1 = op2.Global(1, 0, dtype=np.uint32)
+        g2 = op2.Global(1, 3, dtype=np.uint32)
+        op2.par_loop(k, s1, d1(op2.IdentityMap, op2.READ), g1(op2.INC), g2(op2.INC))
+        assert g1.data[0] == total and g2.data[0] == NELEMS + 3

@@ -64,6 +64,65 @@ def test_iterations(pair):
     assert abs(A.rms.data[0] - B.rms.data[0]) <= RTOL * abs(B.rms.data[0])
 
 
+def test_demo_iterations(pair):
+    """Steps driven as the reference's demo drives them (a new rms Global
+    per update, read back after each step): the per-step rms history and q
+    match the oracle's, and equal the same steps with one reused Global."""
+    A, B = pair
+    hist = [(A.demo_iteration(), B.demo_iteration()) for _ in range(4)]
+    for a, b in hist:
+        assert abs(a - b) <= RTOL * abs(b)
+    assert rel_err(A.p_q.data_ro, B.p_q.data_ro) <= RTOL
+    q = A.p_q.data_ro.copy()
+    A.save_soln()
+    for _ in range(2):
+        A.adt_calc()
+        A.res_calc()
+        A.bres_calc()
+        A.rms.data = 0.0
+        A.update()
+    want = float(np.sqrt(A.rms.data[0] / A.cells.size))
+    A.p_q.data[:] = q
+    assert A.demo_iteration() == want
+
+
+def test_demo_step_in_graph(hip):
+    """The demo's step with its new rms Globals captured as one graph: a
+    Global created inside the captured function stores its initial value
+    in stream order (no host copy), so every replay starts its rms from 0.
+    Each replay, with eager steps of a second airfoil between replays,
+    equals the eager demo step bit for bit."""
+    from pyop2_amd import hip as H
+    mesh = meshes.QuadMesh(80, 40, seed=4)
+    q0 = airfoil.initial_q(mesh.ncell, seed=6)
+    A = airfoil.Airfoil(hip, mesh, q0=q0)
+    B = airfoil.Airfoil(hip, mesh, q0=q0, consts=False)
+    made = []
+
+    def step():
+        A.save_soln()
+        for _ in range(2):
+            A.adt_calc()
+            A.res_calc()
+            A.bres_calc()
+            made.append(hip.Global(1, 0.0, np.float64, 'rms'))
+            A.update(made[-1])
+
+    step()
+    B.demo_iteration()
+    g = H.capture(step)
+    for _ in range(3):
+        g.replay()
+        want = B.demo_iteration()
+        # a replay leaves host states as the capture did: the values of
+        # the captured Global and of q are read from the device
+        made[-1]._state = H.DEVICE
+        A.p_q._state = H.DEVICE
+        assert float(np.sqrt(made[-1].data[0] / mesh.ncell)) == want
+        assert np.array_equal(A.p_q.data_ro, B.p_q.data_ro)
+    A.remove_consts()
+
+
 def test_repeatable(hip):
     """Two runs from the same inputs are bit-identical (no atomics)."""
     mesh = meshes.QuadMesh(64, 40, seed=2)
