@@ -45,7 +45,8 @@ def parse():
     p.add_argument('--ny', type=int, default=600)
     p.add_argument('--soa', type=int, default=0,
                    help='store q, q_old and res as structure-of-arrays Dats (1)')
-    p.add_argument('--graph', type=int, default=1, help='replay each step as a hipGraph')
+    p.add_argument('--graph', type=int, default=0,
+                   help='replay each step as a hipGraph (default: par_loops called eagerly)')
     p.add_argument('--cold', type=int, default=1,
                    help='time res_calc and adt_calc cache-cold too (0: skip)')
     p.add_argument('--cpu-seconds', type=float, default=15.0,

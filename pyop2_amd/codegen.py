@@ -32,6 +32,8 @@ fixed order combines them with the Global's value: INC adds, MIN/MAX take
 the min/max (reference semantics: pyop2/base.py:216-248).
 """
 
+import re
+
 import numpy as np
 
 from .base import IterationIndex, READ, WRITE, RW, INC, MIN, MAX, canonical
@@ -168,8 +170,13 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
 '''
 
     def __init__(self, kernel, extents, args, consts, mode, block_size, prefetch=1,
-                 stage_reads=True, stage_writes=True, unroll=1, owner_stage=()):
+                 stage_reads=True, stage_writes=True, unroll=1, owner_stage=(),
+                 rows_stage=False):
         assert mode in ('stream', 'coloured', 'owner', 'rows')
+        # rows template: the rows its elements read through maps are staged
+        # in LDS per block (distinct targets once) and read through local
+        # positions, instead of gathered per element
+        self.rows_stage = bool(rows_stage) and mode == 'rows'
         # owner write-back: 1 issues the loads locating a thread's first
         # target before the elements run (they overlap), 0 after them; 2
         # also loads that target's old value early
@@ -229,7 +236,12 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             roles += [('own_tp', None), ('own_xoff', None), ('own_exec', None),
                       ('own_contrib', None), ('own_c16', None), ('own_pos', None),
                       ('own_k', None), ('own_nnodes', None), ('own_slice', None)]
-            roles += [('rows_xmap', m) for m in self.rows_maps()]
+            if self.rows_stage:
+                roles.append(('own_stage', None))
+                for m in self.rows_maps():
+                    roles += [('rows_sn', m), ('rows_so', m), ('rows_xl', m)]
+            else:
+                roles += [('rows_xmap', m) for m in self.rows_maps()]
         for a in self.infos:
             for _, role in a.params():
                 roles.append((role, a.pos))
@@ -405,9 +417,9 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                         mloads.append('%s = %s[%s * %d + %d];'
                                       % (mi, *self._mrow(p, E, mslot), a.mdim, a.idx))
                     decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
-                    loads.append('{ const %s* src = d%d + (op2_i64)%s * %d; '
+                    loads.append('{ const %s* src = %s + (op2_i64)%s * %d; '
                                  'for (int k = 0; k < %d; ++k) %s[k] = src[k]; }'
-                                 % (t, p, mi, a.cdim, a.cdim, n('rl')))
+                                 % (t, self._rows_src(p), mi, a.cdim, a.cdim, n('rl')))
                     call_args.append(n('rl'))
                 elif local:
                     decls.append('%s %s[%d];' % (t, n('rl'), a.cdim))
@@ -443,10 +455,10 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                         mloads.append('for (int j = 0; j < %d; ++j) %s[j] = %s[%s * %d + j];'
                                       % (a.mdim, mi, *self._mrow(p, E, mslot), a.mdim))
                     decls.append('%s %s[%d][%d];' % (t, n('rl'), a.mdim, a.cdim))
-                    loads.append('for (int j = 0; j < %d; ++j) { const %s* src = d%d + '
+                    loads.append('for (int j = 0; j < %d; ++j) { const %s* src = %s + '
                                  '(op2_i64)%s[j] * %d; for (int k = 0; k < %d; ++k) '
                                  '%s[j][k] = src[k]; }'
-                                 % (a.mdim, t, p, mi, a.cdim, a.cdim, n('rl')))
+                                 % (a.mdim, t, self._rows_src(p), mi, a.cdim, a.cdim, n('rl')))
                     compute.append('for (int j = 0; j < %d; ++j) %s[j] = %s[j];'
                                    % (a.mdim, n('v'), n('rl')))
                 elif local:
@@ -522,7 +534,13 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         ``E``, or the exec-ordered map rows at slot ``mslot``."""
         if mslot is None:
             return 'm%d' % self._m(p), E
-        return 'xm%d' % self._m(p), '(op2_i64)(%s)' % mslot
+        return ('xl%d' if self.rows_stage else 'xm%d') % self._m(p), '(op2_i64)(%s)' % mslot
+
+    def _rows_src(self, p):
+        """Where argument ``p``'s rows are read through its map entries: the
+        Dat, or (staged rows template) the block's LDS region, the entries
+        then being local positions."""
+        return 's_g%d' % self._stage_region(p) if self.rows_stage else 'd%d' % p
 
     def rows_maps(self):
         """Map parameter positions whose entries the rows template reads
@@ -530,11 +548,42 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         mloads = self._element_body(split='maps', mslot='0')[1]
         out = []
         for l in mloads:
-            i = l.index('= xm') + 4
-            m = int(l[i:].split('[')[0])
+            m = int(re.search(r'= x[ml](\d+)\[', l).group(1))
             if m not in out:
                 out.append(m)
         return out
+
+    def rows_staged(self):
+        """The LDS regions of a staged rows template: (argument position,
+        map parameter, values per row, C type) per distinct (Dat, map) its
+        register rows come through, in argument order."""
+        out, seen = [], {}
+        for a in self._by_map_rows():
+            k = (id(a.arg.data), self._m(a.pos))
+            if k not in seen:
+                seen[k] = a.pos
+                out.append((a.pos, self._m(a.pos), a.cdim, a.ctype))
+        return out
+
+    def _stage_region(self, p):
+        """The staged region argument ``p`` reads (the first argument's of
+        its Dat and map)."""
+        a = self.infos[p]
+        for a2 in self._by_map_rows():
+            if a2.arg.data is a.arg.data and self._m(a2.pos) == self._m(p):
+                return a2.pos
+        raise KeyError(p)
+
+    def _by_map_rows(self):
+        """Arguments whose rows the element body of a template with
+        ``split='maps'`` loads into registers through map entries."""
+        written = set(id(a.arg.data) for a in self.infos
+                      if a.kind not in ('gbl_read', 'gbl_red') and a.access is not READ)
+        return [a for a in self.infos
+                if a.kind in ('ind', 'vec') and a.access is READ and not a.soa and
+                id(a.arg.data) not in written and
+                a.cdim * (a.mdim if a.kind == 'vec' else 1) <= 16 and
+                not (a.kind == 'ind' and a.pos in self.owner_stage)]
 
     def _m(self, p):
         """Position of the map parameter argument ``p`` reads."""
@@ -1190,14 +1239,100 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
       if (i < nsl) d%(p)d[(op2_i64)base + i] = sv[u];
     }
   }""" % dict(sub, U=8)
-        out = [head, slice_pre, ind(decls, 2), ind(decls_n, 2), ind(ml('ex0 + threadIdx.x'), 2),
-               ind(loads, 2), ind(ml('ex0 + %d + threadIdx.x' % bs), 2), slice_in,
-               '  for (int c0 = 0; c0 < nex; c0 += %(bs)d) {' % sub]
+        if self.rows_stage:
+            # the staged rows go to LDS before the first chunk's rows are read
+            out = [head, self._rows_stage_fill(sub), slice_pre, ind(decls, 2), ind(decls_n, 2),
+                   ind(ml('ex0 + threadIdx.x'), 2), self._rows_stage_copy(sub), slice_in,
+                   '  __syncthreads();', ind(loads, 2),
+                   ind(ml('ex0 + %d + threadIdx.x' % bs), 2),
+                   '  for (int c0 = 0; c0 < nex; c0 += %(bs)d) {' % sub]
+        else:
+            out = [head, slice_pre, ind(decls, 2), ind(decls_n, 2),
+                   ind(ml('ex0 + threadIdx.x'), 2), ind(loads, 2),
+                   ind(ml('ex0 + %d + threadIdx.x' % bs), 2), slice_in,
+                   '  for (int c0 = 0; c0 < nex; c0 += %(bs)d) {' % sub]
         out += step('e', loads_n, compute, 'lm%d' % p)
         out += ['    c0 += %(bs)d;' % sub, '    if (c0 >= nex) break;']
         out += step('e_n', loads, compute_n, 'lm%d_n' % p)
         out += ['  }', slice_out]
         return out
+
+    # staged rows one thread loads per pass
+    ROWS_STAGE_UNROLL = 4
+
+    def _rows_stage_fill(self, sub):
+        """Staged rows template, block start: the LDS regions' pointers and
+        the first pass of the staged rows' loads (node numbers, then rows,
+        into registers; the LDS stores come in ``_rows_stage_copy``)."""
+        regs = self.rows_staged()
+        U, bs = self.ROWS_STAGE_UNROLL, self.bs
+        lines = ['  // staged rows: per (Dat, map) the rows of the targets the block\'s',
+                 '  // elements name, at 16-byte aligned LDS offsets after the slice,',
+                 '  // the local matrices and the row offsets (own_stage rows each)',
+                 '  char* s_gb = (char*)op2_lds + (((((op2_i64)own_slice + %d * %d) * (op2_i64)sizeof(%s))'
+                 ' + 4 * ((op2_i64)kk * %d + 1) + 15) & ~(op2_i64)15);'
+                 % (sub['pitch'], bs, sub['t'], sub['rd'])]
+        off = '0'
+        for p, m, c, t in regs:
+            lines.append('  %s* s_g%d = (%s*)(s_gb + %s);' % (t, p, t, off))
+            off = '%s + ((own_stage * %d * (op2_i64)sizeof(%s) + 15) & ~(op2_i64)15)' % (off, c, t)
+        for m in sorted(set(r[1] for r in regs)):
+            lines += ['  const int sg%d0 = so%d[b], sg%dn = so%d[b + 1] - sg%d0;' % (m, m, m, m, m),
+                      '  int sgn%d[%d];' % (m, U),
+                      '#pragma unroll',
+                      '  for (int u = 0; u < %d; ++u) {' % U,
+                      '    const int j = u * %d + (int)threadIdx.x;' % bs,
+                      '    sgn%d[u] = sg%dn > 0 ? sn%d[sg%d0 + (j < sg%dn ? j : sg%dn - 1)] : 0;'
+                      % (m, m, m, m, m, m),
+                      '  }']
+        for p, m, c, t in regs:
+            lines += ['  %s sgv%d[%d][%d];' % (t, p, U, c),
+                      '#pragma unroll',
+                      '  for (int u = 0; u < %d; ++u)' % U,
+                      '#pragma unroll',
+                      '    for (int k = 0; k < %d; ++k) sgv%d[u][k] = d%d[(op2_i64)sgn%d[u] * %d + k];'
+                      % (c, p, p, m, c)]
+        return '\n'.join(lines)
+
+    def _rows_stage_copy(self, sub):
+        """Staged rows template: the first pass's rows stored to LDS, then
+        the further passes (node numbers, rows, stores)."""
+        regs = self.rows_staged()
+        U, bs = self.ROWS_STAGE_UNROLL, self.bs
+        lines = []
+
+        def store(j0):
+            out = []
+            for p, m, c, t in regs:
+                out += ['#pragma unroll',
+                        '  for (int u = 0; u < %d; ++u) {' % U,
+                        '    const int j = %s + u * %d + (int)threadIdx.x;' % (j0, bs),
+                        '    if (j < sg%dn)' % m,
+                        '#pragma unroll',
+                        '      for (int k = 0; k < %d; ++k) s_g%d[j * %d + k] = sgv%d[u][k];'
+                        % (c, p, c, p),
+                        '  }']
+            return out
+        lines += store('0')
+        maps = sorted(set(r[1] for r in regs))
+        cond = ' || '.join('j0 < sg%dn' % m for m in maps)
+        lines += ['  for (int j0 = %d; %s; j0 += %d) {' % (U * bs, cond, U * bs)]
+        for m in maps:
+            lines += ['#pragma unroll',
+                      '    for (int u = 0; u < %d; ++u) {' % U,
+                      '      const int j = j0 + u * %d + (int)threadIdx.x;' % bs,
+                      '      sgn%d[u] = sg%dn > 0 ? sn%d[sg%d0 + (j < sg%dn ? j : sg%dn - 1)] : 0;'
+                      % (m, m, m, m, m, m),
+                      '    }']
+        for p, m, c, t in regs:
+            lines += ['#pragma unroll',
+                      '    for (int u = 0; u < %d; ++u)' % U,
+                      '#pragma unroll',
+                      '      for (int k = 0; k < %d; ++k) sgv%d[u][k] = d%d[(op2_i64)sgn%d[u] * %d + k];'
+                      % (c, p, p, m, c)]
+        lines += ['  ' + l for l in store('j0')]
+        lines.append('  }')
+        return '\n'.join(lines)
 
     def rows_pitch(self):
         """Values per slot of a row block's local matrices (odd)."""
@@ -1205,12 +1340,18 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         ent = self.extents[0] * self.extents[1] * a.rdim * a.cdim_m
         return ent | 1
 
-    def rows_lds_bytes(self, k, slice_max):
+    def rows_lds_bytes(self, k, slice_max, stage_max=0):
         """LDS bytes of a row block: its CSR slice, the chunk's local
-        matrices and its rows' slice offsets."""
+        matrices and its rows' slice offsets; staged, also ``stage_max``
+        rows per staged region."""
         a = [x for x in self.infos if x.kind == 'mat'][0]
         isz = np.dtype(a.arg.data.dtype).itemsize
-        return isz * (slice_max + self.rows_pitch() * self.bs) + 4 * (k * a.rdim + 1)
+        n = isz * (slice_max + self.rows_pitch() * self.bs) + 4 * (k * a.rdim + 1)
+        if not self.rows_stage:
+            return n
+        r16 = lambda x: (x + 15) & ~15
+        return r16(n) + sum(r16(stage_max * c * np.dtype(self.infos[q].arg.data.dtype).itemsize)
+                            for q, _, c, _ in self.rows_staged())
 
     def _stage_offset(self, pos):
         """Values per owned row staged before argument ``pos`` (all of
@@ -1254,7 +1395,13 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                      'const int* __restrict__ p_exec', 'const void* __restrict__ p_contrib',
                      'op2_i64 own_c16', 'const unsigned* __restrict__ p_pos', 'op2_i64 own_k',
                      'op2_i64 own_nnodes', 'op2_i64 own_slice']
-            decl += ['const int* __restrict__ xm%d' % m for m in self.rows_maps()]
+            if self.rows_stage:
+                decl.append('op2_i64 own_stage')
+                for m in self.rows_maps():
+                    decl += ['const int* __restrict__ sn%d' % m, 'const int* __restrict__ so%d' % m,
+                             'const unsigned short* __restrict__ xl%d' % m]
+            else:
+                decl += ['const int* __restrict__ xm%d' % m for m in self.rows_maps()]
         for a in self.infos:
             decl += [d for d, _ in a.params()]
         if self.has_mats:

@@ -977,6 +977,7 @@ class RowsMatPlan(object):
                                                       ctypes.c_void_p(stream())), 'pack codes')
         self._dev = d
         self._xmaps = {}
+        self._stages = {}
         self._sizes = {'tp': nnodes + 1, 'xoff': self.nblk + 1, 'exec': ninc, 'code': ninc,
                        'pos': ninc * W}
 
@@ -989,12 +990,20 @@ class RowsMatPlan(object):
         return self._dev
 
     def stage(self, m):
-        """Per-block staging lists through map ``m`` (each block's distinct
-        targets and each slot's positions in them, the reference's ind_map
-        and loc_map) are switched off: their device build faulted on its
-        first GPU runs and the cause is not established.  The rows template
-        gathers element rows from the Dats directly."""
-        raise RuntimeError("row-block staging lists are switched off")
+        """Per-block staging lists through map ``m`` (the reference's
+        per-block ind_map and loc_map, plan.pyx:112-212), cached per map
+        array: (sn, so, xl, smax).  Block b's distinct targets, ascending,
+        are sn[so[b]:so[b+1]]; xl holds per exec slot the positions of its
+        element's m.dim targets in its block's list (uint16, padded like
+        exec with position 0); smax is the longest list.  None when a list
+        has more than 65536 targets.  Built with torch's sort-based unique on
+        the (block, target) pairs of the exec-ordered map rows."""
+        dv = m._device_values()
+        key = dv.data_ptr()
+        if key not in self._stages:
+            self._stages[key] = rows_stage_lists(self._dev['xoff'][:self.nblk + 1],
+                                                 self.xmap(m)[:self.sum_exec], ROWS_EXEC_PAD)
+        return self._stages[key]
 
     def xmap(self, m):
         """The rows of map ``m`` (device values) in exec-list order, padded
@@ -1020,6 +1029,34 @@ class RowsMatPlan(object):
         return a.astype(np.int64)
 
 
+def rows_stage_lists(xoff, rows, pad):
+    """Staging lists of row blocks (the reference's per-block ind_map and
+    loc_map, plan.pyx:112-212) from ``xoff`` (nblk + 1 exec offsets) and
+    ``rows`` (the exec slots' map rows, sum_exec x dim int32), on their
+    device: (sn, so, xl, smax) with block b's distinct targets ascending in
+    sn[so[b]:so[b+1]], per slot the positions of its targets in its block's
+    list in xl (int16 holding uint16, ``pad`` zero slots appended), smax the
+    longest list (at least 2); None when a list is longer than 65536.  A
+    sort-based unique of the (block, target) pairs."""
+    dev = rows.device
+    S, dim = rows.shape
+    nb = xoff.numel() - 1
+    xoff = xoff.long()
+    blk = torch.repeat_interleave(torch.arange(nb, device=dev), xoff[1:] - xoff[:-1],
+                                  output_size=S)
+    pairs = (blk.unsqueeze(1) << 32) | rows.long()
+    uniq, inv = torch.unique(pairs.view(-1), sorted=True, return_inverse=True)
+    del pairs
+    so = torch.searchsorted(uniq >> 32, torch.arange(nb + 1, device=dev))
+    smax = int((so[1:] - so[:-1]).max()) if nb else 0
+    if smax > 0x10000:
+        return None
+    xl = torch.zeros((S + pad) * dim, dtype=torch.int32, device=dev)
+    xl[:S * dim] = (inv.view(S, dim) - so[blk].unsqueeze(1)).view(-1)
+    return ((uniq & 0xffffffff).to(torch.int32), so.to(torch.int32), xl.to(torch.int16),
+            max(smax, 2))
+
+
 _rows_cache = {}
 # largest local matrix (values per element) the rows template takes
 ROWS_MAX_ENTRIES = 16
@@ -1033,17 +1070,19 @@ ROWS_RECORDS_PAD = 16
 ROWS_LDS_LIMIT = 52 * 1024
 
 
-def get_rows_plan(iterset, n, args, bs, lds_of, lds_limit):
+def get_rows_plan(iterset, n, args, bs, lds_of, lds_limit, stage_maps=()):
     """Cached row-block plan of an owner Mat assembly, or None: ``k`` row
     nodes per block (``hip_mat_rows_k``, default the block size), halved
-    until the block's LDS ``lds_of(k, largest slice)`` fits ``lds_limit``;
-    None when no k from 8 up fits or a row's column positions do not fit a
-    byte."""
+    until the block's LDS ``lds_of(k, largest slice, longest staging list
+    through stage_maps)`` fits ``lds_limit``; None when no k from 8 up fits,
+    a row's column positions do not fit a byte or a staging list is too
+    long."""
     a = [x for x in args if x._is_mat][0]
     sp = a.data.sparsity
     e0, e1 = a.map[0].dim, a.map[1].dim
     k = min(int(cfg.get('hip_mat_rows_k') or bs), bs)
-    key = (id(iterset), n, id(a.map[0]), id(a.map[1]), id(sp), k, bs, lds_limit)
+    key = (id(iterset), n, id(a.map[0]), id(a.map[1]), id(sp), k, bs, lds_limit,
+           tuple(id(m) for m in stage_maps))
     hit = _rows_cache.get(key)
     if hit is not None and hit[0] is sp:
         return hit[1]
@@ -1051,11 +1090,14 @@ def get_rows_plan(iterset, n, args, bs, lds_of, lds_limit):
     rank = None if lay.identity else _device_perm(lay)
     plan = None
     # the chunk's local matrices alone may not fit
-    while k >= 8 and lds_of(k, 0) <= lds_limit:
+    while k >= 8 and lds_of(k, 0, 0) <= lds_limit:
         p = RowsMatPlan(n, a, e0, e1, k, rank)
         if not p.fits:
             break
-        if lds_of(k, p.max_slice) <= lds_limit:
+        st = [p.stage(m) for m in stage_maps]
+        if any(x is None for x in st):
+            break
+        if lds_of(k, p.max_slice, max([x[3] for x in st] or [0])) <= lds_limit:
             plan = p
             break
         k //= 2
@@ -1342,11 +1384,13 @@ class _LaunchEntry(object):
         if mode == 'rows':
             limit = min(int(cfg.get('hip_mat_rows_lds') or ROWS_LDS_LIMIT),
                         _state.get('lds_per_block') or 65536)
-            plan = get_rows_plan(iterset, n, args, bs, src.rows_lds_bytes, limit)
+            smaps = [args[m].map for m in src.rows_maps()] if src.rows_stage else []
+            plan = get_rows_plan(iterset, n, args, bs, src.rows_lds_bytes, limit, smaps)
             if plan is None:
                 raise _NoOwnerPlan()
             nparts = plan.nblk
-            lds = src.rows_lds_bytes(plan.k, plan.max_slice)
+            stage_max = max([plan.stage(m)[3] for m in smaps] or [0])
+            lds = src.rows_lds_bytes(plan.k, plan.max_slice, stage_max)
             grid_stream = 0
         elif mode == 'owner':
             slot_bytes = src.owner_slot_bytes()
@@ -1396,6 +1440,8 @@ class _LaunchEntry(object):
                 vals.append(_ptr(a.data._dev))
             elif role == 'rows_xmap':
                 vals.append(_ptr(plan.xmap(a.map)))
+            elif role in ('rows_sn', 'rows_so', 'rows_xl'):
+                vals.append(_ptr(plan.stage(a.map)[('rows_sn', 'rows_so', 'rows_xl').index(role)]))
             elif role == 'map':
                 vals.append(_ptr(a.map._device_values()))
             elif role == 'mat_vals':
@@ -1424,7 +1470,8 @@ class _LaunchEntry(object):
                 names = {'own_tp': 'tp', 'own_xoff': 'xoff', 'own_exec': 'exec',
                          'own_contrib': 'code', 'own_pos': 'pos'}
                 scal = {'start': 0, 'count': plan.nblk, 'own_c16': int(plan.c16),
-                        'own_k': plan.k, 'own_nnodes': plan.nnodes, 'own_slice': plan.max_slice}
+                        'own_k': plan.k, 'own_nnodes': plan.nnodes, 'own_slice': plan.max_slice,
+                        'own_stage': stage_max}
                 argv = [scal[r] if r in scal else _ptr(d[names[r]]) if r in names else r
                         for r in vals]
                 if plan.nblk:
@@ -1632,10 +1679,11 @@ class ParLoop(base.ParLoop):
         sw = bool(cfg.get('hip_stage_writes', True))
         un = int(cfg.get('hip_stream_unroll', 1))
         st = self._owner_stage() if mode == 'owner' and cfg.get('hip_owner_stage') else ()
+        rs = mode == 'rows' and bool(cfg.get('hip_mat_rows_stage'))
         mat = any(a._is_mat for a in self.args)
         mw = int(cfg.get('hip_owner_waves', 0)) if mode == 'owner' and not mat else 0
         alias = self._map_alias()
-        key = key + (mode, bs, pf, sr, sw, un, st, mw, alias)
+        key = key + (mode, bs, pf, sr, sw, un, st, mw, alias, rs)
         c = _loop_cache.get(key)
         if c is not None:
             return key, c.source
@@ -1643,7 +1691,7 @@ class ParLoop(base.ParLoop):
         if src is None:
             src = LoopSource(self.kernel, self._it_space.extents, self.args, consts, mode, bs,
                              prefetch=pf, stage_reads=sr, stage_writes=sw, unroll=un,
-                             owner_stage=st)
+                             owner_stage=st, rows_stage=rs)
             src.min_waves = mw
             src.map_of = {i: r for i, r in enumerate(alias) if r != i}
             if len(_source_cache) >= _CALL_CACHE_MAX:
@@ -1759,7 +1807,7 @@ _VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill', 'hi
                  'hip_stage_reads', 'hip_stage_writes', 'hip_stream_unroll',
                  'hip_stream_blocks', 'hip_owner_prefetch', 'hip_owner_waves',
                  'hip_owner_mat_lds', 'hip_colouring', 'hip_mat_rows', 'hip_mat_rows_block_size',
-                 'hip_mat_rows_k', 'hip_mat_rows_lds')
+                 'hip_mat_rows_k', 'hip_mat_rows_lds', 'hip_mat_rows_stage')
 
 
 def _written_buffers(pl):
