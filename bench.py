@@ -32,6 +32,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # bytes written between the launches of the cache-cold loop timings
 FLUSH_BYTES = 512 << 20
+# GPU clock cycles of torch.cuda._sleep before a singly timed launch (about
+# 110 us on MI355X: longer than the host takes to enqueue one par_loop)
+LAUNCH_DELAY_CYCLES = 1000
 
 
 def parse():
@@ -219,20 +222,25 @@ def main():
         nat.lib.op2hip_event_elapsed_ms(a, b, ctypes.byref(ms))
         return ms.value / 1e3 / (reps * nrep)
 
-    def cold_avg(loop, flush, nrep=40):
-        # one loop after a write and then a read of `flush` (larger than the
-        # L2s and the 256 MB Infinity Cache together), so its inputs come
-        # from HBM; the read leaves the caches holding clean lines, so the
-        # loop does not pay for writing back the flush's own data
-        g = hip.capture(loop)
-        g.replay()
+    def single_avg(loop, flush=None, nrep=40):
+        # one loop launched on its own, eagerly, behind a GPU-side delay
+        # long enough for the host to have enqueued the launch before it
+        # ends, so the events time the kernel and its dispatch (not the
+        # host's launch, nor a graph launch's latency); with `flush`, after
+        # a write and then a read of `flush` (larger than the L2s and the
+        # 256 MB Infinity Cache together), so its inputs come from HBM (the
+        # read leaves the caches holding clean lines, so the loop does not
+        # pay for writing back the flush's own data)
+        loop()
         torch.cuda.synchronize()
         evs = [(mkev(), mkev()) for _ in range(nrep)]
         for i, (a, b) in enumerate(evs):
-            flush.fill_(float(i))
-            flush.sum()
+            if flush is not None:
+                flush.fill_(float(i))
+                flush.sum()
+            torch.cuda._sleep(LAUNCH_DELAY_CYCLES)
             nat.lib.op2hip_event_record(a, ctypes.c_void_p(s))
-            g.replay()
+            loop()
             nat.lib.op2hip_event_record(b, ctypes.c_void_p(s))
         torch.cuda.synchronize()
         tot, ms = 0.0, ctypes.c_float()
@@ -267,9 +275,11 @@ def main():
         cold = {'flush_bytes': FLUSH_BYTES}
         for name, loop in (('res_calc', A.res_calc), ('adt_calc', A.adt_calc)):
             hot_s = res_avg_s if name == 'res_calc' else hot_avg(loop)
-            cold_s = cold_avg(loop, flush)
+            single_s = single_avg(loop)
+            cold_s = single_avg(loop, flush)
             cold[name] = {'unique_bytes': int(ub[name]), 'hot_us': round(hot_s * 1e6, 2),
                           'hot_frac': round(ub[name] / hot_s / 1e9 / HBM_PEAK_GBS, 4),
+                          'single_us': round(single_s * 1e6, 2),
                           'cold_us': round(cold_s * 1e6, 2),
                           'cold_frac': round(ub[name] / cold_s / 1e9 / HBM_PEAK_GBS, 4)}
         del flush
