@@ -91,15 +91,29 @@ def _ptr(t):
     return t.data_ptr() if t is not None else 0
 
 
+# uploads up to this size go through pinned staging and do not wait
+H2D_ASYNC_MAX = 64 << 20
+
+
 def _h2d(dev, host):
-    """Copy a host numpy array's bytes into a device byte tensor."""
-    if host.nbytes == 0:
+    """Copy a host numpy array's bytes into a device byte tensor, in stream
+    order.  Up to H2D_ASYNC_MAX bytes the bytes are first copied into a
+    pinned staging buffer (torch's caching host allocator keeps it from
+    reuse until the copy has run), so the call returns without waiting for
+    the stream; larger arrays are copied synchronously."""
+    n = host.nbytes
+    if n == 0:
         return
     if torch.cuda.is_current_stream_capturing():
         raise RuntimeError("host data reached the device during graph capture: run the "
                            "captured function eagerly once before capturing it")
-    src = torch.from_numpy(np.ascontiguousarray(host).reshape(-1).view(np.uint8))
-    dev[:host.nbytes].copy_(src)
+    flat = np.ascontiguousarray(host).reshape(-1).view(np.uint8)
+    if n <= H2D_ASYNC_MAX:
+        stage = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+        stage.numpy()[:] = flat
+        dev[:n].copy_(stage, non_blocking=True)
+    else:
+        dev[:n].copy_(torch.from_numpy(flat))
 
 
 def _d2h(host, dev):

@@ -165,18 +165,28 @@ class Server(object):
 
     def run(self, requests):
         """Serve ``requests`` (arrival times relative to the start); returns
-        them with admitted/done times, iterations and reasons filled in."""
+        them with admitted/done times, iterations and reasons filled in.
+
+        Each engine step enqueues the batch's iterations first and then, while
+        the device runs them, admits the requests that have arrived and
+        enqueues their assembly (behind the iterations on the same stream);
+        they join the batch at the next step."""
         from . import hip
         s = hip.stream()
         queue = sorted(requests, key=lambda r: r.arrival)
         qi = 0
         active = []          # (request, slot, system)
+        pending = []         # admitted and assembled, joining at the next step
         t0 = time.perf_counter()
-        while qi < len(queue) or active:
+        while qi < len(queue) or active or pending:
+            active += pending
+            pending = []
+            if active:
+                self.cg.load([a[2] for a in active], s)
+                self.cg.step(self.iters, s)
             now = time.perf_counter() - t0
-            admitted = []
             while qi < len(queue) and queue[qi].arrival <= now and \
-                    len(active) + len(admitted) < self.max_active:
+                    len(active) + len(pending) < self.max_active:
                 req = queue[qi]
                 rm = self.pool[req.mesh_id]
                 slot = self._take_slot(rm)
@@ -184,17 +194,17 @@ class Server(object):
                     break        # HBM budget full: wait for a slot to free up
                 req.admitted = now
                 slot.assemble(kappa_of(req, rm.n))
-                admitted.append((req, slot, slot.system(self.rtol, self.maxit)))
+                pending.append((req, slot, slot.system(self.rtol, self.maxit)))
                 qi += 1
-            if not active and not admitted:
-                if qi < len(queue) and queue[qi].arrival > now:
-                    time.sleep(min(queue[qi].arrival - now, 0.002))
-                    continue
-                if qi < len(queue):
-                    raise RuntimeError("request %d does not fit the HBM budget" % queue[qi].rid)
-            active += admitted
-            self.cg.load([a[2] for a in active], s)
-            self.cg.step(self.iters, s)
+            if not active:
+                if not pending:
+                    if qi < len(queue) and queue[qi].arrival > now:
+                        time.sleep(min(queue[qi].arrival - now, 0.002))
+                        continue
+                    if qi < len(queue):
+                        raise RuntimeError("request %d does not fit the HBM budget"
+                                           % queue[qi].rid)
+                continue
             out = self.cg.read(s)
             keep = []
             for (req, slot, _), st in zip(active, out):
