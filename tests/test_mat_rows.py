@@ -317,3 +317,33 @@ def test_rows_staged_assembly_bitexact(hip):
                        x(mv, W.READ))
             out.append(np.array(A.values))
         np.testing.assert_array_equal(out[1], out[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('stage', [False, True])
+@pytest.mark.parametrize('nel', [1, 37])
+def test_rows_edge_sets(hip, stage, nel):
+    """Row blocks on edge-case sets: one element, and a node set with nodes
+    no element names (empty rows, and blocks of them only); Mat values
+    bit-exact to the oracle, with and without staged rows.  (A Sparsity
+    needs populated maps, so an empty element set cannot be assembled, as
+    in the reference: base.py:1164-1168.)"""
+    from pyop2_amd import configuration as cfg
+    mesh = meshes.TriMesh(6, seed=5)
+    en = mesh.elem_node[:nel].astype(np.int32)
+    nv = mesh.nvert + 70          # 70 nodes no element names
+    coords = np.vstack([mesh.coords, np.random.default_rng(2).random((70, 2))])
+    out = []
+    for W in (seq, hip):
+        nodes = W.Set(nv)
+        vn = W.Set(nv, 2)
+        el = W.Set(nel)
+        m = W.Map(el, nodes, 3, en.reshape(-1, 3))
+        mv = W.Map(el, vn, 3, en.reshape(-1, 3))
+        x = W.Dat(vn, coords.copy(), np.float64)
+        A = W.Mat(W.Sparsity((m, m)), np.float64)
+        with cfg.override(hip_mat_rows_stage=stage):
+            W.par_loop(W.Kernel(fem.MASS_TRI, 'mass_tri'), el(3, 3),
+                       A((m[W.i[0]], m[W.i[1]]), W.INC), x(mv, W.READ))
+        out.append(np.array(A.values))
+    np.testing.assert_array_equal(out[1], out[0])
