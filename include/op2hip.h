@@ -303,8 +303,7 @@ typedef struct op2hip_cg_system {
   double* x;
   double* inv_diag;
   double* r;
-  double* z;
-  double* p;
+  double* zp; /* z and p interleaved: zp[2i] = z_i, zp[2i+1] = p_i; 16-byte aligned */
   double* w;
   int row0;
   int n;
@@ -331,7 +330,8 @@ int op2hip_cgb_history(void* h, int k, double* out, void* stream);
 int op2hip_cgb_sizeof_system(void);
 int op2hip_cgb_chunk_rows(void);
 /* One solve (replaces Solver.solve, pyop2/petsc_base.py:182-220, for one
- * CSR on the device): work holds 5*n device doubles; its, rnorm and reason
+ * CSR on the device): work holds 5*n device doubles, 16-byte aligned (D^-1,
+ * r, z and p interleaved over 2*n, w); its, rnorm and reason
  * (KSPConvergedReason) are host outputs.  Waits for the stream. */
 int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, const double* b,
                double* x, double* work, double rtol, double atol, double dtol, int maxit,

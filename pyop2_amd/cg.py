@@ -26,7 +26,7 @@ _i = ctypes.c_int
 class CGSystem(ctypes.Structure):
     """Mirror of ``op2hip_cg_system`` (include/op2hip.h)."""
     _fields_ = [('rowptr', _p), ('colidx', _p), ('vals', _p), ('b', _p), ('x', _p),
-                ('inv_diag', _p), ('r', _p), ('z', _p), ('p', _p), ('w', _p),
+                ('inv_diag', _p), ('r', _p), ('zp', _p), ('w', _p),
                 ('row0', _i), ('n', _i), ('chunk0', _i), ('nchunks', _i),
                 ('rtol', _d), ('atol', _d), ('dtol', _d), ('maxit', _i), ('state', _i),
                 ('no_precond', _i), ('beta', _d), ('betaold', _d), ('dpi', _d), ('dpiold', _d), ('dp', _d),
@@ -45,13 +45,19 @@ def chunk_rows():
 
 def make_system(rowptr, colidx, vals, b, x, work, row0, n, rtol, atol, dtol, maxit,
                 jacobi=True):
-    """A NEW system over rows [row0, row0+n).  ``work`` holds 5 device
-    vectors (inv_diag, r, z, p, w) laid out like x (indexed by global row):
-    pass the base addresses of 5 arrays of the global length.  ``jacobi``
-    False runs CG without a preconditioner (PETSc's PCNONE)."""
+    """A NEW system over rows [row0, row0+n).  ``work`` holds the base
+    addresses of the 5 rows of one (5, length) device array of doubles laid
+    out like x (indexed by global row): inv_diag, r, then z and p
+    interleaved over rows 2 and 3 (z_i at 2 i, p_i at 2 i + 1), and w.
+    ``jacobi`` False runs CG without a preconditioner (PETSc's PCNONE)."""
+    pitch = work[1] - work[0]
+    if any(work[j + 1] - work[j] != pitch for j in range(4)) or pitch <= 0:
+        raise ValueError("work must be the rows of one (5, length) array")
+    if work[2] % 16:
+        raise ValueError("work rows must be 16-byte aligned")
     s = CGSystem()
     s.rowptr, s.colidx, s.vals, s.b, s.x = rowptr, colidx, vals, b, x
-    s.inv_diag, s.r, s.z, s.p, s.w = work
+    s.inv_diag, s.r, s.zp, s.w = work[0], work[1], work[2], work[4]
     s.row0, s.n = row0, n
     s.rtol, s.atol, s.dtol, s.maxit = rtol, atol, dtol, maxit
     s.state = SYS_NEW

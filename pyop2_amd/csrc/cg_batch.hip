@@ -9,7 +9,9 @@
 //         x += a p; r -= a w; z = D^-1 r; dp = |z|; test; beta = z.r
 // Four launches per iteration: w = A p with p formed from z and the old p
 // as it is gathered, partial p.w (k_spmv); a (k_alpha); p stored, x, r, z,
-// partials z.z and z.r (k_axpy); dp, the test and beta (k_finish).
+// partials z.z and z.r (k_axpy); dp, the test and beta (k_finish).  z and p
+// are stored interleaved (zp[2 i] = z_i, zp[2 i + 1] = p_i): one 16-byte
+// gather per matrix entry and one load and one store per row in k_axpy.
 //   ttol = max(rtol*|z0|, atol); converged dp <= ttol; diverged
 //   dp >= dtol*|z0|; DIVERGED_ITS after max_it.
 //
@@ -58,8 +60,7 @@ struct op2hip_cg_system {
   double* x;
   double* inv_diag;
   double* r;
-  double* z;
-  double* p;
+  double* zp;  // z and p interleaved: zp[2 i] = z_i, zp[2 i + 1] = p_i (16-byte aligned)
   double* w;
   int row0;
   int n;
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunk
   double* __restrict__ x = s.x;
   double* __restrict__ inv_diag = s.inv_diag;
   double* __restrict__ r = s.r;
-  double* __restrict__ z = s.z;
+  double2* __restrict__ zp = reinterpret_cast<double2*>(s.zp);
   for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) {
     double d = 0.0;
     for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
@@ -123,7 +124,8 @@ __global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunk
     double zi = inv * ri;
     x[i] = 0.0;
     r[i] = ri;
-    z[i] = zi;
+    // p is first formed from z alone; its slot starts at zero
+    zp[i] = make_double2(zi, 0.0);
     zz += zi * zi;
     zr += zi * ri;
   }
@@ -199,7 +201,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
   const int* __restrict__ rowptr = s.rowptr;
   const int* __restrict__ colidx = s.colidx;
   const double* __restrict__ vals = s.vals;
-  const double* __restrict__ p = s.p;
+  const double2* __restrict__ zp = reinterpret_cast<const double2*>(s.zp);
   double* __restrict__ w = s.w;
   // lanes per row from the chunk's mean row length (about two entries per
   // lane: 4 lanes for 2-D P1 rows, 8 for 3-D, 16 for 3-D vector rows); a
@@ -209,28 +211,56 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
   int lanes = 2;
   while (lanes < kLanes && 2 * lanes < mean) lanes <<= 1;
   const int sub = threadIdx.x & (lanes - 1);
-  const double* __restrict__ z = s.z;
+  const int stride = kBlock / lanes;
   // the new direction p' = z (first iteration) or z + (beta/betaold) p,
-  // formed where it is read: the gathered columns here, each row again in
-  // k_axpy, which stores it
+  // formed where it is read: the gathered columns here (one 16-byte load
+  // of z and p per entry), each row again in k_axpy, which stores it
   const bool first = s.its == 0;
   const double bb = first ? 0.0 : s.beta / s.betaold;
   double acc = 0.0;
-  for (int i = lo + (int)threadIdx.x / lanes; i < hi; i += kBlock / lanes) {
-    const int kb = rowptr[i], ke = rowptr[i + 1];
-    double v = 0.0;
-    if (first) {
-      for (int k = kb + sub; k < ke; k += lanes) v += vals[k] * z[colidx[k]];
-    } else {
-      for (int k = kb + sub; k < ke; k += lanes) {
-        const int c = colidx[k];
-        v += vals[k] * (z[c] + bb * p[c]);
-      }
+  // Two rows per lane group (i0 and i0 + stride) and two entries of each
+  // per lane are loaded together, so four gathers are in flight per lane.
+  // A lane still adds its entries k, k + lanes, k + 2 lanes, ... of a row
+  // in that order and the rows enter acc in ascending order, as one row at
+  // a time would: the sums do not change.  Entries past a row's end load
+  // row 0's z and p (present in every batch) and are not added.
+  for (int i0 = lo + (int)threadIdx.x / lanes; i0 < hi; i0 += 2 * stride) {
+    const int i1 = i0 + stride;
+    const bool has1 = i1 < hi;
+    const int kb0 = rowptr[i0], ke0 = rowptr[i0 + 1];
+    const int kb1 = has1 ? rowptr[i1] : 0, ke1 = has1 ? rowptr[i1 + 1] : 0;
+    double v0 = 0.0, v1 = 0.0;
+    for (int t = sub;; t += 2 * lanes) {
+      const int ka = kb0 + t, kc = kb1 + t;
+      const bool a0 = ka < ke0, a1 = ka + lanes < ke0, c0 = kc < ke1, c1 = kc + lanes < ke1;
+      if (!a0 && !c0) break;
+      const int ca0 = a0 ? colidx[ka] : 0, ca1 = a1 ? colidx[ka + lanes] : 0;
+      const int cc0 = c0 ? colidx[kc] : 0, cc1 = c1 ? colidx[kc + lanes] : 0;
+      const double va0 = a0 ? vals[ka] : 0.0, va1 = a1 ? vals[ka + lanes] : 0.0;
+      const double vc0 = c0 ? vals[kc] : 0.0, vc1 = c1 ? vals[kc + lanes] : 0.0;
+      const double2 qa0 = zp[ca0], qa1 = zp[ca1], qc0 = zp[cc0], qc1 = zp[cc1];
+      const double pa0 = first ? qa0.x : qa0.x + bb * qa0.y;
+      const double pa1 = first ? qa1.x : qa1.x + bb * qa1.y;
+      const double pc0 = first ? qc0.x : qc0.x + bb * qc0.y;
+      const double pc1 = first ? qc1.x : qc1.x + bb * qc1.y;
+      if (a0) v0 += va0 * pa0;
+      if (a1) v0 += va1 * pa1;
+      if (c0) v1 += vc0 * pc0;
+      if (c1) v1 += vc1 * pc1;
     }
-    for (int off = lanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, lanes);
+    for (int off = lanes / 2; off > 0; off >>= 1) {
+      v0 += __shfl_xor(v0, off, lanes);
+      v1 += __shfl_xor(v1, off, lanes);
+    }
     if (sub == 0) {
-      w[i] = v;
-      acc += (first ? z[i] : z[i] + bb * p[i]) * v;
+      w[i0] = v0;
+      const double2 q0 = zp[i0];
+      acc += (first ? q0.x : q0.x + bb * q0.y) * v0;
+      if (has1) {
+        w[i1] = v1;
+        const double2 q1 = zp[i1];
+        acc += (first ? q1.x : q1.x + bb * q1.y) * v1;
+      }
     }
   }
   double t = block_sum<kBlock>(acc, sh);
@@ -272,22 +302,21 @@ __global__ void __launch_bounds__(kBlock) k_axpy(Sys* sys, const ChunkRef* chunk
   double zz = 0.0, zr = 0.0;
   const double a = s.alpha;
   double* __restrict__ x = s.x;
-  double* __restrict__ p = s.p;
   double* __restrict__ r = s.r;
   const double* __restrict__ w = s.w;
   const double* __restrict__ inv_diag = s.inv_diag;
-  double* __restrict__ z = s.z;
+  double2* __restrict__ zp = reinterpret_cast<double2*>(s.zp);
   // k_alpha has counted this iteration: its == 1 on the first
   const bool first = s.its == 1;
   const double bb = first ? 0.0 : s.beta / s.betaold;
   for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) {
-    const double pi = first ? z[i] : z[i] + bb * p[i];
-    p[i] = pi;
+    const double2 q = zp[i];
+    const double pi = first ? q.x : q.x + bb * q.y;
     x[i] = x[i] + a * pi;
     double ri = r[i] - a * w[i];
     r[i] = ri;
     double zi = inv_diag[i] * ri;
-    z[i] = zi;
+    zp[i] = make_double2(zi, pi);
     zz += zi * zi;
     zr += zi * ri;
   }
@@ -415,6 +444,8 @@ int op2hip_cgb_load(void* h, int nsys, const op2hip_cg_system* systems, void* st
   std::vector<ChunkRef> chunks;
   for (int k = 0; k < nsys; ++k) {
     Sys& sy = b->h_sys[k];
+    OP2HIP_REQUIRE(sy.state == SYS_EMPTY || (reinterpret_cast<uintptr_t>(sy.zp) & 15) == 0,
+                   "system %d: zp must be 16-byte aligned", k);
     sy.chunk0 = static_cast<int>(chunks.size());
     // finished and empty slots get no chunks: launches cover live rows only
     sy.nchunks = (sy.state == SYS_EMPTY || sy.state == SYS_DONE) ? 0 : (sy.n + kChunk - 1) / kChunk;
@@ -562,6 +593,8 @@ int op2hip_pcg_noalloc(int n, const int* rowptr, const int* colidx, const double
                        void* stream) {
   OP2HIP_REQUIRE(n >= 0 && rowptr && colidx && vals && b && x && work && its && rnorm && reason,
                  "null argument to op2hip_pcg_noalloc");
+  OP2HIP_REQUIRE((reinterpret_cast<uintptr_t>(work) & 15) == 0,
+                 "op2hip_pcg_noalloc: work must be 16-byte aligned");
   OP2HIP_REQUIRE(n <= kNoAllocMaxRows, "op2hip_pcg_noalloc takes at most %d rows, not %d",
                  kNoAllocMaxRows, n);
   unsigned char* base = nullptr;
@@ -584,8 +617,7 @@ int op2hip_pcg_noalloc(int n, const int* rowptr, const int* colidx, const double
   sy.x = x;
   sy.inv_diag = work;
   sy.r = work + n;
-  sy.z = work + 2 * (size_t)n;
-  sy.p = work + 3 * (size_t)n;
+  sy.zp = work + 2 * (size_t)n;
   sy.w = work + 4 * (size_t)n;
   sy.row0 = 0;
   sy.n = n;
@@ -622,8 +654,8 @@ int op2hip_cgb_chunk_rows(void) { return kChunk; }
 
 
 // One Jacobi-PCG solve (the single-system form of the batch): A (n x n CSR
-// on the device), b, x (device, x overwritten), work (device, 5*n doubles:
-// D^-1, r, z, p, w).  PETSc KSPCG + PCJACOBI semantics as above; its,
+// on the device), b, x (device, x overwritten), work (device, 5*n doubles,
+// 16-byte aligned: D^-1, r, then z and p interleaved over 2*n, then w).  PETSc KSPCG + PCJACOBI semantics as above; its,
 // rnorm (the last preconditioned residual norm) and reason (KSP converged
 // reason) are written to host memory.  Waits for the stream.
 int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, const double* b,
@@ -631,6 +663,8 @@ int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, 
                int* its, double* rnorm, int* reason, void* stream) {
   OP2HIP_REQUIRE(n >= 0 && rowptr && colidx && vals && b && x && work && its && rnorm && reason,
                  "null argument to op2hip_pcg");
+  OP2HIP_REQUIRE((reinterpret_cast<uintptr_t>(work) & 15) == 0,
+                 "op2hip_pcg: work must be 16-byte aligned");
   void* h = nullptr;
   int st = op2hip_cgb_create(1, (n + kChunk - 1) / kChunk + 1, 0, &h);
   if (st != OP2HIP_OK) return st;
@@ -642,8 +676,7 @@ int op2hip_pcg(int n, const int* rowptr, const int* colidx, const double* vals, 
   sy.x = x;
   sy.inv_diag = work;
   sy.r = work + n;
-  sy.z = work + 2 * (size_t)n;
-  sy.p = work + 3 * (size_t)n;
+  sy.zp = work + 2 * (size_t)n;
   sy.w = work + 4 * (size_t)n;
   sy.row0 = 0;
   sy.n = n;

@@ -77,7 +77,7 @@ def test_single_pcg_matches_batch_and_direct(hip):
     assert reason.value > 0 and its.value > 0
     # the batch engine with one system gives the same x bit for bit
     x2 = torch.zeros(n, dtype=torch.float64, device='cuda')
-    w2 = [torch.zeros(n, dtype=torch.float64, device='cuda') for _ in range(5)]
+    w2 = torch.zeros((5, n), dtype=torch.float64, device='cuda')
     sysd = cg.make_system(rp.data_ptr(), ci.data_ptr(), va.data_ptr(), bb.data_ptr(),
                           x2.data_ptr(), [t.data_ptr() for t in w2], 0, n, 1e-12, 1e-50, 1e5,
                           10000)
