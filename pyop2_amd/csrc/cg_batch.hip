@@ -83,6 +83,18 @@ using Sys = op2hip_cg_system;
 
 using op2hip::block_sum;
 
+// The systems' arrays are global memory, but their pointers come out of a
+// descriptor in memory, so the compiler cannot tell their address space and
+// emits flat loads; cast to global, they load with global instructions.
+#define OP2HIP_GLOBAL __attribute__((address_space(1)))
+// z and p of a row (a clang vector type: it loads and stores as one 16-byte
+// access in any address space)
+typedef double zp_pair __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ inline OP2HIP_GLOBAL T* gmem(T* p) {
+  return (OP2HIP_GLOBAL T*)p;
+}
+
 // chunk list entry: (system, chunk index within the system)
 struct ChunkRef {
   int sys, chunk;
@@ -106,14 +118,14 @@ __global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunk
   double zz = 0.0, zr = 0.0;
   // the system's arrays as local pointers: stores through them cannot
   // change the descriptor, so it is read once
-  const int* __restrict__ rowptr = s.rowptr;
-  const int* __restrict__ colidx = s.colidx;
-  const double* __restrict__ vals = s.vals;
-  const double* __restrict__ b = s.b;
-  double* __restrict__ x = s.x;
-  double* __restrict__ inv_diag = s.inv_diag;
-  double* __restrict__ r = s.r;
-  double2* __restrict__ zp = reinterpret_cast<double2*>(s.zp);
+  const OP2HIP_GLOBAL int* __restrict__ rowptr = gmem(s.rowptr);
+  const OP2HIP_GLOBAL int* __restrict__ colidx = gmem(s.colidx);
+  const OP2HIP_GLOBAL double* __restrict__ vals = gmem(s.vals);
+  const OP2HIP_GLOBAL double* __restrict__ b = gmem(s.b);
+  OP2HIP_GLOBAL double* __restrict__ x = gmem(s.x);
+  OP2HIP_GLOBAL double* __restrict__ inv_diag = gmem(s.inv_diag);
+  OP2HIP_GLOBAL double* __restrict__ r = gmem(s.r);
+  OP2HIP_GLOBAL zp_pair* __restrict__ zp = gmem(reinterpret_cast<zp_pair*>(s.zp));
   for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) {
     double d = 0.0;
     for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
@@ -125,7 +137,7 @@ __global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunk
     x[i] = 0.0;
     r[i] = ri;
     // p is first formed from z alone; its slot starts at zero
-    zp[i] = make_double2(zi, 0.0);
+    zp[i] = zp_pair{zi, 0.0};
     zz += zi * zi;
     zr += zi * ri;
   }
@@ -198,11 +210,11 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
   if (s.state != SYS_RUNNING) return;
   int lo, hi;
   chunk_rows(s, cr.chunk, lo, hi);
-  const int* __restrict__ rowptr = s.rowptr;
-  const int* __restrict__ colidx = s.colidx;
-  const double* __restrict__ vals = s.vals;
-  const double2* __restrict__ zp = reinterpret_cast<const double2*>(s.zp);
-  double* __restrict__ w = s.w;
+  const OP2HIP_GLOBAL int* __restrict__ rowptr = gmem(s.rowptr);
+  const OP2HIP_GLOBAL int* __restrict__ colidx = gmem(s.colidx);
+  const OP2HIP_GLOBAL double* __restrict__ vals = gmem(s.vals);
+  const OP2HIP_GLOBAL zp_pair* __restrict__ zp = gmem(reinterpret_cast<const zp_pair*>(s.zp));
+  OP2HIP_GLOBAL double* __restrict__ w = gmem(s.w);
   // lanes per row from the chunk's mean row length (about two entries per
   // lane: 4 lanes for 2-D P1 rows, 8 for 3-D, 16 for 3-D vector rows); a
   // property of the chunk alone, so a system sums alike in any batch
@@ -238,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
       const int cc0 = c0 ? colidx[kc] : 0, cc1 = c1 ? colidx[kc + lanes] : 0;
       const double va0 = a0 ? vals[ka] : 0.0, va1 = a1 ? vals[ka + lanes] : 0.0;
       const double vc0 = c0 ? vals[kc] : 0.0, vc1 = c1 ? vals[kc + lanes] : 0.0;
-      const double2 qa0 = zp[ca0], qa1 = zp[ca1], qc0 = zp[cc0], qc1 = zp[cc1];
+      const zp_pair qa0 = zp[ca0], qa1 = zp[ca1], qc0 = zp[cc0], qc1 = zp[cc1];
       const double pa0 = first ? qa0.x : qa0.x + bb * qa0.y;
       const double pa1 = first ? qa1.x : qa1.x + bb * qa1.y;
       const double pc0 = first ? qc0.x : qc0.x + bb * qc0.y;
@@ -254,11 +266,11 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
     }
     if (sub == 0) {
       w[i0] = v0;
-      const double2 q0 = zp[i0];
+      const zp_pair q0 = zp[i0];
       acc += (first ? q0.x : q0.x + bb * q0.y) * v0;
       if (has1) {
         w[i1] = v1;
-        const double2 q1 = zp[i1];
+        const zp_pair q1 = zp[i1];
         acc += (first ? q1.x : q1.x + bb * q1.y) * v1;
       }
     }
@@ -301,22 +313,22 @@ __global__ void __launch_bounds__(kBlock) k_axpy(Sys* sys, const ChunkRef* chunk
   chunk_rows(s, cr.chunk, lo, hi);
   double zz = 0.0, zr = 0.0;
   const double a = s.alpha;
-  double* __restrict__ x = s.x;
-  double* __restrict__ r = s.r;
-  const double* __restrict__ w = s.w;
-  const double* __restrict__ inv_diag = s.inv_diag;
-  double2* __restrict__ zp = reinterpret_cast<double2*>(s.zp);
+  OP2HIP_GLOBAL double* __restrict__ x = gmem(s.x);
+  OP2HIP_GLOBAL double* __restrict__ r = gmem(s.r);
+  const OP2HIP_GLOBAL double* __restrict__ w = gmem(s.w);
+  const OP2HIP_GLOBAL double* __restrict__ inv_diag = gmem(s.inv_diag);
+  OP2HIP_GLOBAL zp_pair* __restrict__ zp = gmem(reinterpret_cast<zp_pair*>(s.zp));
   // k_alpha has counted this iteration: its == 1 on the first
   const bool first = s.its == 1;
   const double bb = first ? 0.0 : s.beta / s.betaold;
   for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) {
-    const double2 q = zp[i];
+    const zp_pair q = zp[i];
     const double pi = first ? q.x : q.x + bb * q.y;
     x[i] = x[i] + a * pi;
     double ri = r[i] - a * w[i];
     r[i] = ri;
     double zi = inv_diag[i] * ri;
-    zp[i] = make_double2(zi, pi);
+    zp[i] = zp_pair{zi, pi};
     zz += zi * zi;
     zr += zi * ri;
   }
