@@ -167,6 +167,68 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
   }
   return (lo <= hi && colidx[lo] == col) ? lo : -1;
 }
+// A deferred FP64 Global reduction (the combine pass of an earlier loop)
+// run by one extra workgroup of this loop's launch, its first: op 0 INC,
+// 1 MIN, 2 MAX.  The workgroup's waves stand in for the 16 wavefronts of
+// the 1024-thread combine kernel (its thread t combines partials t,
+// t + 1024, t + 2048, t + 3072, then t + 4096, ... in that order; then a
+// shuffle tree per wavefront and the 16 wavefront values in order), so the
+// Global gets that kernel's value bit for bit.  A wave takes two of its
+// stand-in wavefronts at a time, their loads issued together (more would
+// raise the register count of every loop kernel).
+__device__ inline double op2_pend_op(int op, double a, double b) {
+  return op == 0 ? a + b : op == 1 ? (b < a ? b : a) : (b > a ? b : a);
+}
+__device__ inline void op2_pending_reduce(op2_i64 nparts, const double* __restrict__ r,
+                                          double* g, op2_i64 info) {
+  __shared__ double op2_pend_s[16];
+  const int cdim = (int)(info & 0xffff), op = (int)(info >> 16);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int k = 0; k < cdim; ++k) {
+    for (int vg = wv; vg < 16; vg += 2 * nw) {
+      double v[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) v[j] = op == 0 ? 0.0 : g[k];
+      // partial counts are workgroup counts: 32-bit offsets
+      const int np = (int)nparts;
+      for (int q00 = 0; q00 < np; q00 += 4096) {
+        double x[2][4];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int vw = vg + j * nw;
+            const int q = q00 + vw * 64 + lane + u * 1024;
+            x[j][u] = (vw < 16 && q < np) ? r[q * cdim + k] : 0.0;
+          }
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int vw = vg + j * nw;
+            const int q = q00 + vw * 64 + lane + u * 1024;
+            if (vw < 16 && q < np) v[j] = op2_pend_op(op, v[j], x[j][u]);
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int vw = vg + j * nw;
+        if (vw < 16) {
+          double t = v[j];
+          for (int off = 32; off > 0; off >>= 1) t = op2_pend_op(op, t, op2_shfl_xor(t, off));
+          if (lane == 0) op2_pend_s[vw] = t;
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double v = op2_pend_s[0];
+      for (int w = 1; w < 16; ++w) v = op2_pend_op(op, v, op2_pend_s[w]);
+      g[k] = op2_pend_op(op, g[k], v);
+    }
+    __syncthreads();
+  }
+}
 '''
 
     def __init__(self, kernel, extents, args, consts, mode, block_size, prefetch=1,
@@ -247,6 +309,8 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                 roles.append((role, a.pos))
         if self.has_mats:
             roles.append(('err_flag', None))
+        # a deferred Global reduction of an earlier loop (zero when none)
+        roles += [('pend_n', None), ('pend_part', None), ('pend_out', None), ('pend_info', None)]
         return roles
 
     def reduce_roles(self):
@@ -681,7 +745,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         u = self.stream_unroll()
         # workgroups are dealt to the 8 XCDs round-robin: give each XCD a
         # contiguous run of chunks, so neighbouring chunks share its L2
-        out.append('  const int nbk = (int)gridDim.x, bid = (int)blockIdx.x;')
+        out.append('  const int nbk = (int)gridDim.x - (pend_n > 0 ? 1 : 0), bid = op2_bid;')
         out.append('  const int xcd = bid & 7, qx = nbk >> 3, rx = nbk & 7;')
         out.append('  const op2_i64 lb = xcd * qx + (xcd < rx ? xcd : rx) + (bid >> 3);')
         out.append('  const op2_i64 stride = (op2_i64)nbk * %d;' % (bs * u))
@@ -790,7 +854,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         out = ['  extern __shared__ %s op2_lds[];' % t,
                ind(self._red_init(), 2),
                '''  // consecutive owner blocks (neighbouring target ranges) go to one XCD
-  const int nb = (int)count, bid = blockIdx.x;
+  const int nb = (int)count, bid = op2_bid;
   const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
   const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
   // slots [0, rn) run elements rlo + slot (primary); the rest are listed
@@ -914,7 +978,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
 }''' % dict(sub, val=val)]
         out = ['  extern __shared__ double op2_lds[];',
                ind(self._red_init(), 2),
-               '''  const int nb = (int)count, bid = blockIdx.x;
+               '''  const int nb = (int)count, bid = op2_bid;
   const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
   const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
   // slots [0, rn) run elements rlo + slot (primary); the rest are listed
@@ -1064,7 +1128,7 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
                'op': '=' if a.access is WRITE else '+='}
         head = """  extern __shared__ double op2_lds[];
   // consecutive blocks (neighbouring row ranges) go to one XCD
-  const int nb = (int)count, bid = blockIdx.x;
+  const int nb = (int)count, bid = op2_bid;
   const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
   const int b = (int)start + xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
   const int kk = (int)own_k, t0 = b * kk;
@@ -1406,6 +1470,8 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
             decl += [d for d, _ in a.params()]
         if self.has_mats:
             decl.append('int* __restrict__ err_flag')
+        decl += ['op2_i64 pend_n', 'const double* __restrict__ pend_part', 'double* pend_out',
+                 'op2_i64 pend_info']
         return decl
 
     def source(self):
@@ -1419,6 +1485,14 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         bounds = '%d, %d' % (bs, self.min_waves) if self.min_waves else '%d' % bs
         out.append('extern "C" __global__ void __launch_bounds__(%s) %s(\n    %s) {'
                    % (bounds, self.entry, ',\n    '.join(self._params_decl())))
+        out.append('''  // with a deferred reduction the launch has one more workgroup, the
+  // first (dispatched first, it overlaps the loop's), which runs it; the
+  // loop's own workgroups count from the next
+  if (pend_n > 0 && blockIdx.x == 0) {
+    op2_pending_reduce(pend_n, pend_part, pend_out, pend_info);
+    return;
+  }
+  const int op2_bid = (int)blockIdx.x - (pend_n > 0 ? 1 : 0);''')
         if self.mode == 'rows':
             out += self._rows_body()
             out.append('}')
@@ -1426,14 +1500,14 @@ __device__ inline int op2_csr_find(const int* __restrict__ rowptr, const int* __
         decls, compute = self._element_body()
         if self.mode == 'stream':
             out += self._stream_body(decls, compute)
-            out.append(ind(self._red_finish('blockIdx.x'), 2))
+            out.append(ind(self._red_finish('op2_bid'), 2))
         elif self.mode == 'owner':
             out += self._owner_body(decls, compute)
         else:
             out.append(ind(self._red_init(), 2))
             out.append('''  // blocks of one colour are spread over the 8 XCDs in contiguous
   // chunks, so neighbouring plan blocks share an L2
-  const int nb = (int)count, bid = blockIdx.x;
+  const int nb = (int)count, bid = op2_bid;
   const int xcd = bid & 7, q = nb >> 3, rr = nb & 7;
   const int lb = xcd * q + (xcd < rr ? xcd : rr) + (bid >> 3);
   const int b = p_blkmap[start + lb];

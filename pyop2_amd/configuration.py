@@ -46,6 +46,7 @@ DEFAULTS = {
     'hip_mat_rows_k': 64,         # row nodes per row block (at most the workgroup size)
     'hip_mat_rows_lds': 0,        # LDS bytes a row block may use (0: 52 KB)
     'hip_mat_rows_stage': False,  # row blocks stage their elements' rows in LDS
+    'hip_defer_reductions': True,  # an FP64 Global's combine pass rides on the next launch
     'hip_owner_waves': 0,         # waves per SIMD owner loops are compiled for (0: compiler's choice)
     'hip_device_plan': True,      # owner plans built on the device (False: the host build)
     'hip_layout': True,           # locality renumbering of device storage

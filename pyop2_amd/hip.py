@@ -36,7 +36,7 @@ from . import layout
 from . import autotune
 from .base import (READ, WRITE, RW, INC, MIN, MAX, IdentityMap, IterationIndex, i,
                    IterationSpace, Halo, Access)
-from .codegen import LoopSource
+from .codegen import LoopSource, _mode as _access_mode
 from .exceptions import MatTypeError, DatTypeError
 from .logger import debug, warning
 from .mpi import MPI
@@ -248,6 +248,12 @@ class Global(_DeviceData, base.Global):
     def __init__(self, *args, **kwargs):
         base.Global.__init__(self, *args, **kwargs)
         self._dev_init()
+
+    def _sync_to_host(self):
+        p = _pending[0]
+        if p is not None and p.glob is self:
+            flush()
+        _DeviceData._sync_to_host(self)
 
     def _upload(self):
         # a value of up to 256 bytes reaches the device in stream order, as
@@ -1324,7 +1330,7 @@ class _LaunchEntry(object):
     valid for as long as the entry is cached."""
 
     __slots__ = ('loop', 'launches', 'to_device', 'written', 'dats', 'globals_red', 'mats',
-                 'consts', 'keep')
+                 'consts', 'keep', 'has_reduce', 'red_index', 'red_info', 'red_glob')
 
     def rebindable(self, args):
         """A copy whose Global arguments can be exchanged for others of
@@ -1333,7 +1339,8 @@ class _LaunchEntry(object):
         pointer.  Reference-style drivers build a new Global per call
         (demo/airfoil.py:124-130: ``rms = op2.Global(...)`` in the loop)."""
         c = object.__new__(_LaunchEntry)
-        for k in ('loop', 'consts', 'dats', 'mats', 'globals_red'):
+        for k in ('loop', 'consts', 'dats', 'mats', 'globals_red', 'has_reduce', 'red_index',
+                  'red_info', 'red_glob'):
             setattr(c, k, getattr(self, k, None))
         c.launches = [(fn, grid, bs, ((ctypes.c_uint64 * n)(*arr), n), lds)
                       for fn, grid, bs, (arr, n), lds in self.launches]
@@ -1362,6 +1369,8 @@ class _LaunchEntry(object):
                 self.launches[li][3][0][ai] = ptr
             self.to_device = [new if d is cur else d for d in self.to_device]
             self.written = [new if d is cur else d for d in self.written]
+            if self.red_glob is cur:
+                self.red_glob = new
             # the bound Globals stay alive through globals_red / to_device
             self.globals_red[pos] = (new, where)
 
@@ -1470,6 +1479,8 @@ class _LaunchEntry(object):
                 vals.append(_ptr(a.map[1]._device_values()))
             elif role == 'err_flag':
                 vals.append(_ptr([x for x in args if x._is_mat][0].data._err))
+            elif role.startswith('pend_'):
+                vals.append(0)
             else:
                 raise RuntimeError("unknown parameter role %s" % role)
         self.keep.append(partials)
@@ -1541,6 +1552,21 @@ class _LaunchEntry(object):
                         rargv.append(_ptr(partials[pos]))
                 launches.append((loop.reduce_fn, 1, 1024, _argv(rargv), 0))
         self.launches = launches
+        # one FP64 Global reduction: its combine pass is deferred to the
+        # next launch (see _Pending)
+        self.red_index = self.red_info = self.red_glob = None
+        reds = [a for a in args if a._is_global_reduction]
+        if loop.reduce_fn is not None and launches and launches[-1][0] is loop.reduce_fn:
+            self.has_reduce = True
+            op = _red_mode(reds[0].access) if len(reds) == 1 else None
+            if (op is not None and reds[0].data.dtype == np.float64 and
+                    reds[0].data.cdim < 0x10000 and nparts * reds[0].data.cdim < 2 ** 31 and
+                    cfg.get('hip_defer_reductions', True)):
+                self.red_index = len(launches) - 1
+                self.red_info = reds[0].data.cdim | (op << 16)
+                self.red_glob = reds[0].data
+        else:
+            self.has_reduce = loop.reduce_fn is not None
         self.written = [a.data for a in args if (a._is_dat and a.access is not READ) or
                         a._is_global_reduction]
         self.dats = [a.data for a in args if a._is_dat]
@@ -1567,8 +1593,26 @@ class _LaunchEntry(object):
         self.prepare()
         sp = ctypes.c_void_p(stream())
         launch = nat.lib.op2hip_launch
-        for fn, grid, bs, (arr, n), lds in self.launches:
-            st = launch(fn, grid, bs, lds, sp, arr, n)
+        pend = _pending[0]
+        if pend is not None and (self.has_reduce or any(d is pend.glob for d in self.to_device)):
+            # this loop reduces (its partials, its Global) or uses the
+            # deferred Global: the combine pass runs first, on its own
+            flush()
+            pend = None
+        red, fn0 = self.red_index, self.loop.fn
+        for li, (fn, grid, bs, (arr, n), lds) in enumerate(self.launches):
+            if li == red:
+                _pending[0] = _Pending(fn, arr, self.red_info, self.red_glob, self)
+                continue
+            if pend is not None and fn is fn0 and bs % 64 == 0:
+                # the deferred combine pass rides along as one more workgroup
+                arr[n - 4], arr[n - 3], arr[n - 2], arr[n - 1] = (
+                    pend.argv[0], pend.argv[2], pend.argv[1], pend.info)
+                st = launch(fn, grid + 1, bs, lds, sp, arr, n)
+                arr[n - 4] = arr[n - 3] = arr[n - 2] = arr[n - 1] = 0
+                _pending[0] = pend = None
+            else:
+                st = launch(fn, grid, bs, lds, sp, arr, n)
             if st != 0:
                 nat.check(st, 'kernel launch')
         for d in self.written:
@@ -1578,6 +1622,43 @@ class _LaunchEntry(object):
             a = d._data
             if a.flags.writeable:
                 maybe_setflags(a, write=False)
+
+
+class _Pending(object):
+    """The combine pass of an FP64 Global reduction whose loop has run: the
+    next loop launch that neither reduces nor names the Global runs it in
+    one extra workgroup (codegen's op2_pending_reduce, in the combine
+    kernel's order), else :func:`flush` launches the combine kernel."""
+
+    __slots__ = ('fn', 'argv', 'info', 'glob', 'keep')
+
+    def __init__(self, fn, arr, info, glob, keep):
+        # a copy of the combine launch's arguments (nparts, Global,
+        # partials): a later call of the same loop may rebind its Global
+        self.fn, self.argv, self.info = fn, (ctypes.c_uint64 * 3)(*arr[:3]), info
+        self.glob, self.keep = glob, keep
+
+
+# at most one deferred reduction, in stream order behind the launches made
+_pending = [None]
+
+
+def _red_mode(access):
+    """Combine code of a Global reduction's access: INC 0, MIN 1, MAX 2."""
+    return {'INC': 0, 'MIN': 1, 'MAX': 2}.get(_access_mode(access))
+
+
+def flush():
+    """Launch the deferred Global reduction, if any (on the current
+    stream).  Reading a Global's value, capturing a graph and loops that use
+    the Global or reduce themselves do this first."""
+    p = _pending[0]
+    if p is None:
+        return
+    _pending[0] = None
+    st = nat.lib.op2hip_launch(p.fn, 1, 1024, 0, ctypes.c_void_p(stream()), p.argv, 3)
+    if st != 0:
+        nat.check(st, 'kernel launch')
 
 
 def _argv(values):
@@ -1821,7 +1902,8 @@ _VARIANT_KEYS = ('hip_block_size', 'hip_owner_block_size', 'hip_owner_fill', 'hi
                  'hip_stage_reads', 'hip_stage_writes', 'hip_stream_unroll',
                  'hip_stream_blocks', 'hip_owner_prefetch', 'hip_owner_waves',
                  'hip_owner_mat_lds', 'hip_colouring', 'hip_mat_rows', 'hip_mat_rows_block_size',
-                 'hip_mat_rows_k', 'hip_mat_rows_lds', 'hip_mat_rows_stage')
+                 'hip_mat_rows_k', 'hip_mat_rows_lds', 'hip_mat_rows_stage',
+                 'hip_defer_reductions')
 
 
 def _written_buffers(pl):
@@ -1865,6 +1947,7 @@ def _trial(pl, mode, params, budget_s=2e-3):
         reps = int(min(50, max(3, budget_s / max(one, 1e-7))))
         return min(timed(reps) for _ in range(3))
     finally:
+        flush()
         torch.cuda.synchronize()
         for d, t, st in saved:
             buf = d._dev_vals if isinstance(d, Mat) else d._dev
@@ -1950,8 +2033,11 @@ class Graph(object):
 
     def __init__(self, fn):
         self._g = torch.cuda.CUDAGraph()
+        flush()
         with torch.cuda.graph(self._g):
             fn()
+            # a reduction deferred by the last loop is part of the graph
+            flush()
 
     def replay(self):
         self._g.replay()
