@@ -1591,10 +1591,12 @@ class _LaunchEntry(object):
 
     def run(self):
         self.prepare()
-        sp = ctypes.c_void_p(stream())
+        cur = stream()
+        sp = ctypes.c_void_p(cur)
         launch = nat.lib.op2hip_launch
         pend = _pending[0]
-        if pend is not None and (self.has_reduce or any(d is pend.glob for d in self.to_device)):
+        if pend is not None and (self.has_reduce or pend.stream != cur or
+                                 any(d is pend.glob for d in self.to_device)):
             # this loop reduces (its partials, its Global) or uses the
             # deferred Global: the combine pass runs first, on its own
             flush()
@@ -1602,7 +1604,7 @@ class _LaunchEntry(object):
         red, fn0 = self.red_index, self.loop.fn
         for li, (fn, grid, bs, (arr, n), lds) in enumerate(self.launches):
             if li == red:
-                _pending[0] = _Pending(fn, arr, self.red_info, self.red_glob, self)
+                _pending[0] = _Pending(fn, arr, self.red_info, self.red_glob, self, cur)
                 continue
             if pend is not None and fn is fn0 and bs % 64 == 0:
                 # the deferred combine pass rides along as one more workgroup
@@ -1630,13 +1632,13 @@ class _Pending(object):
     one extra workgroup (codegen's op2_pending_reduce, in the combine
     kernel's order), else :func:`flush` launches the combine kernel."""
 
-    __slots__ = ('fn', 'argv', 'info', 'glob', 'keep')
+    __slots__ = ('fn', 'argv', 'info', 'glob', 'keep', 'stream')
 
-    def __init__(self, fn, arr, info, glob, keep):
+    def __init__(self, fn, arr, info, glob, keep, strm):
         # a copy of the combine launch's arguments (nparts, Global,
         # partials): a later call of the same loop may rebind its Global
         self.fn, self.argv, self.info = fn, (ctypes.c_uint64 * 3)(*arr[:3]), info
-        self.glob, self.keep = glob, keep
+        self.glob, self.keep, self.stream = glob, keep, strm
 
 
 # at most one deferred reduction, in stream order behind the launches made
@@ -1649,14 +1651,15 @@ def _red_mode(access):
 
 
 def flush():
-    """Launch the deferred Global reduction, if any (on the current
-    stream).  Reading a Global's value, capturing a graph and loops that use
-    the Global or reduce themselves do this first."""
+    """Launch the deferred Global reduction, if any, on the stream its loop
+    ran on.  Reading a Global's value, capturing a graph, a launch on
+    another stream and loops that use the Global or reduce themselves do
+    this first."""
     p = _pending[0]
     if p is None:
         return
     _pending[0] = None
-    st = nat.lib.op2hip_launch(p.fn, 1, 1024, 0, ctypes.c_void_p(stream()), p.argv, 3)
+    st = nat.lib.op2hip_launch(p.fn, 1, 1024, 0, ctypes.c_void_p(p.stream), p.argv, 3)
     if st != 0:
         nat.check(st, 'kernel launch')
 
