@@ -234,6 +234,25 @@ int op2hip_mat_rows_build(int nelems, int nnodes, const int* rmap, int rarity, i
 /* code32[0, n) narrowed to uint16 into out (every code below 65536). */
 int op2hip_mat_rows_pack16(const int* code32, long long n, unsigned short* out, void* stream);
 
+/* Per-block staging lists of a plan: replaces the staging pass of
+ * pyop2/plan.pyx:112-212 (per block and indirect map, numpy.unique of the
+ * targets its elements name: ind_map, loc_map, ind_sizes, ind_offs).  Block
+ * b's elements are elems[xoff[b] .. xoff[b+1]) (device); its slots are
+ * xoff[b]*dim .. xoff[b+1]*dim of ind_map and loc_map (ind_offs = xoff*dim):
+ * ind_map holds its sorted distinct targets (ind_sizes[b] of them) from the
+ * start of that range, loc_map per slot the position of the slot's target
+ * in that list.  maxkeys bounds (xoff[b+1]-xoff[b])*dim and may be at most
+ * op2hip_stage_max_keys() (LDS per workgroup); *err (device) gets bit 1 for
+ * a block beyond maxkeys or nslots.  Enqueues only. */
+int op2hip_stage_max_keys(void);
+int op2hip_stage_lists(int nblocks, const int* xoff, const int* elems, const int* map, int dim,
+                       int maxkeys, long long nslots, int* ind_map, unsigned short* loc_map,
+                       int* ind_sizes, int* err, void* stream);
+/* The lists end to end: so[nblocks+1] the exclusive scan of ind_sizes,
+ * sn[so[b] ..] block b's list.  Enqueues only. */
+int op2hip_stage_compact(int nblocks, const int* xoff, int dim, const int* ind_sizes,
+                         const int* ind_map, int* so, int* sn, void* stream);
+
 /* ---- data layout ----------------------------------------------------------------
  * No counterpart in the reference, which stores every Set in user order.
  * Orderings that renumber a Set's device storage for locality (user-visible

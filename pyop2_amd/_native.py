@@ -112,6 +112,11 @@ _SIGS = {
                                       c_void, c_void, c_void, c_int, c_int, c_int, c_void, c_ll,
                                       c_void, c_void, c_void, c_void, c_void, P(c_ll), c_void]),
     'op2hip_mat_rows_pack16': (c_int, [c_void, c_ll, c_void, c_void]),
+    'op2hip_stage_max_keys': (c_int, []),
+    'op2hip_stage_lists': (c_int, [c_int, c_void, c_void, c_void, c_int, c_int, c_ll, c_void,
+                                   c_void, c_void, c_void, c_void]),
+    'op2hip_stage_compact': (c_int, [c_int, c_void, c_int, c_void, c_void, c_void, c_void,
+                                     c_void]),
     'op2hip_mat_program': (c_int, [c_int, c_void, c_void, c_void, c_void, c_void, c_void, c_void,
                                    c_void, c_int,
                                    c_void, c_int, c_int, c_int, c_void, c_void, c_int, c_int,

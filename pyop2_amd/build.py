@@ -17,7 +17,7 @@ ARCH = os.environ.get('OP2HIP_ARCH', 'gfx950')
 
 SOURCES = ['runtime.hip', 'plan.cpp', 'plan_device.hip', 'plan_colour.hip', 'sparsity.hip',
            'linalg.hip',
-           'cg_batch.hip', 'krylov.hip']
+           'cg_batch.hip', 'krylov.hip', 'staging.hip']
 
 # Device code is built without packed FP32 instructions (some of their forms
 # have given wrong results on this hardware) and without FP contraction, so

@@ -633,6 +633,35 @@ class Plan(object):
                 self._dev[name] = d
         return self._dev
 
+    def staging(self, m):
+        """The reference plan's staging lists for map ``m`` (a hip Map on this
+        plan's iteration set; plan.pyx:112-212, ind_map / loc_map / ind_sizes
+        / ind_offs), built on the device (:func:`stage_lists`) for the plan's
+        blocks: a dict of host arrays ``ind_map`` (block b's distinct targets
+        ascending at ``ind_offs[b]``, ``ind_sizes[b]`` of them), ``loc_map``
+        (per element slot of the plan's ``elems`` and map column, the
+        position of its target in its block's list) and ``ind_offs``
+        (offset * arity), in device numbering."""
+        off = self.offset.astype(np.int64)
+        ne = self.nelems.astype(np.int64)
+        dim = m.dim
+        if self.nblocks and (off[0] != 0 or (off[1:] != off[:-1] + ne[:-1]).any()):
+            raise RuntimeError("plan blocks are not consecutive runs of elems")
+        xoff = np.concatenate([off, [off[-1] + ne[-1]] if self.nblocks else [0]]).astype(np.int32)
+        if int(xoff[-1]) > len(self.elems):
+            raise RuntimeError("plan blocks run past its elements")
+        elems = self.device_arrays()['elems']
+        st = stage_lists(torch.from_numpy(xoff).to(device()), elems,
+                         m._device_values().view(torch.int32), dim, int(xoff[-1]) * dim,
+                         int(ne.max()) * dim if self.nblocks else 0)
+        if st is None:
+            raise RuntimeError("a plan block names more targets than the staging kernel holds")
+        ind_map, sizes = st[4], st[5]
+        nsl = int(xoff[-1]) * dim
+        return {'ind_map': ind_map[:nsl].cpu().numpy(),
+                'loc_map': st[2][:nsl].cpu().numpy().view(np.uint16),
+                'ind_sizes': sizes.cpu().numpy(), 'ind_offs': xoff[:-1].astype(np.int64) * dim}
+
     def verify(self):
         """True when the plan is conflict-free for its race maps."""
         ok = ctypes.c_int()
@@ -1016,13 +1045,15 @@ class RowsMatPlan(object):
         are sn[so[b]:so[b+1]]; xl holds per exec slot the positions of its
         element's m.dim targets in its block's list (uint16, padded like
         exec with position 0); smax is the longest list.  None when a list
-        has more than 65536 targets.  Built with torch's sort-based unique on
-        the (block, target) pairs of the exec-ordered map rows."""
+        has more than op2hip_stage_max_keys() keys.  Built on the device by
+        :func:`stage_lists`."""
         dv = m._device_values()
         key = dv.data_ptr()
         if key not in self._stages:
-            self._stages[key] = rows_stage_lists(self._dev['xoff'][:self.nblk + 1],
-                                                 self.xmap(m)[:self.sum_exec], ROWS_EXEC_PAD)
+            st = stage_lists(self._dev['xoff'][:self.nblk + 1], self._dev['exec'],
+                             dv.view(torch.int32), m.dim, self.sum_exec * m.dim,
+                             self.max_exec * m.dim, ROWS_EXEC_PAD)
+            self._stages[key] = None if st is None else st[:4]
         return self._stages[key]
 
     def xmap(self, m):
@@ -1049,32 +1080,38 @@ class RowsMatPlan(object):
         return a.astype(np.int64)
 
 
-def rows_stage_lists(xoff, rows, pad):
-    """Staging lists of row blocks (the reference's per-block ind_map and
-    loc_map, plan.pyx:112-212) from ``xoff`` (nblk + 1 exec offsets) and
-    ``rows`` (the exec slots' map rows, sum_exec x dim int32), on their
-    device: (sn, so, xl, smax) with block b's distinct targets ascending in
-    sn[so[b]:so[b+1]], per slot the positions of its targets in its block's
-    list in xl (int16 holding uint16, ``pad`` zero slots appended), smax the
-    longest list (at least 2); None when a list is longer than 65536.  A
-    sort-based unique of the (block, target) pairs."""
-    dev = rows.device
-    S, dim = rows.shape
+def stage_lists(xoff, elems, mvals, dim, nslots, maxkeys, pad=0):
+    """Per-block staging lists on the device (csrc/staging.hip; the
+    reference's ind_map / loc_map, plan.pyx:112-212): block b runs the
+    elements ``elems[xoff[b]:xoff[b+1]]`` (device int32 tensors; ``xoff``
+    has nblocks + 1 ascending entries, ``elems`` at least ``nslots / dim``)
+    and names targets through ``mvals`` (device int32 map values, arity
+    ``dim``).  Returns (sn, so, xl, smax, ind_map, ind_sizes): block b's
+    distinct targets ascending are sn[so[b]:so[b+1]] (also ind_map from
+    xoff[b] * dim, ind_sizes[b] of them); xl holds per slot (xoff[b] * dim
+    + e * dim + j) the position of its target in its block's list (int16
+    holding uint16, ``pad`` zero slots appended); smax is the longest list
+    (at least 2).  None when a block has more than ``maxkeys`` keys or
+    ``maxkeys`` exceeds what the device's LDS holds."""
     nb = xoff.numel() - 1
-    xoff = xoff.long()
-    blk = torch.repeat_interleave(torch.arange(nb, device=dev), xoff[1:] - xoff[:-1],
-                                  output_size=S)
-    pairs = (blk.unsqueeze(1) << 32) | rows.long()
-    uniq, inv = torch.unique(pairs.view(-1), sorted=True, return_inverse=True)
-    del pairs
-    so = torch.searchsorted(uniq >> 32, torch.arange(nb + 1, device=dev))
-    smax = int((so[1:] - so[:-1]).max()) if nb else 0
-    if smax > 0x10000:
+    if maxkeys > nat.lib.op2hip_stage_max_keys():
         return None
-    xl = torch.zeros((S + pad) * dim, dtype=torch.int32, device=dev)
-    xl[:S * dim] = (inv.view(S, dim) - so[blk].unsqueeze(1)).view(-1)
-    return ((uniq & 0xffffffff).to(torch.int32), so.to(torch.int32), xl.to(torch.int16),
-            max(smax, 2))
+    dev = xoff.device
+    i32 = lambda m: torch.zeros(max(m, 1), dtype=torch.int32, device=dev)
+    ind_map, sizes, err = i32(nslots), i32(nb), i32(1)
+    xl = torch.zeros(nslots + pad * dim, dtype=torch.int16, device=dev)
+    so, sn = i32(nb + 1), i32(nslots)
+    P = lambda t: ctypes.c_void_p(_ptr(t))
+    s = ctypes.c_void_p(stream())
+    nat.check(nat.lib.op2hip_stage_lists(nb, P(xoff), P(elems), P(mvals), dim, maxkeys, nslots,
+                                         P(ind_map), P(xl), P(sizes), P(err), s),
+              'staging lists')
+    nat.check(nat.lib.op2hip_stage_compact(nb, P(xoff), dim, P(sizes), P(ind_map), P(so), P(sn),
+                                           s), 'staging compaction')
+    if int(err[0]):
+        return None
+    smax = int(sizes[:nb].max()) if nb else 0
+    return sn, so, xl, max(smax, 2), ind_map, sizes[:nb]
 
 
 _rows_cache = {}
