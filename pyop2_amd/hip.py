@@ -253,6 +253,15 @@ class Global(_DeviceData, base.Global):
         p = _pending[0]
         if p is not None and p.glob is self:
             flush()
+        a = self._data
+        if (getattr(self, '_state', HOST) == DEVICE and a.nbytes <= 256 and
+                a.flags['C_CONTIGUOUS']):
+            # a small value comes back through pinned memory a kernel writes
+            nat.check(nat.lib.op2hip_load_small(ctypes.c_void_p(a.ctypes.data),
+                                                ctypes.c_void_p(_ptr(self._dev)), a.nbytes,
+                                                ctypes.c_void_p(stream())), 'Global read')
+            self._state = BOTH
+            return
         _DeviceData._sync_to_host(self)
 
     def _upload(self):
