@@ -65,10 +65,6 @@ int op2hip_set_global(void* dptr, const void* src, size_t bytes, void* stream);
  * reference follows with a stream synchronize per reduction
  * (pyop2/cuda.py:353-356). */
 int op2hip_store_small(void* dptr, const void* src, size_t bytes, void* stream);
-/* Up to 256 bytes at dptr into host dst after the work in stream (a
- * Global's value read on the host, base.py:928-977 Global.data): a kernel
- * writes them into pinned host memory, then the stream is drained. */
-int op2hip_load_small(void* dst, const void* dptr, size_t bytes, void* stream);
 /* Compile (hiprtc, cached), load and resolve `entry` in one call; the
  * module belongs to the caller.  The JITModule compile+lookup of
  * pyop2/cuda.py:602-651 as one entry point. */

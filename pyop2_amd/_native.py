@@ -68,7 +68,6 @@ _SIGS = {
     'op2hip_module_global': (c_int, [c_void, c_char_p, P(c_void), P(c_size)]),
     'op2hip_set_global': (c_int, [c_void, c_void, c_size, c_void]),
     'op2hip_store_small': (c_int, [c_void, c_void, c_size, c_void]),
-    'op2hip_load_small': (c_int, [c_void, c_void, c_size, c_void]),
     'op2hip_launch': (c_int, [c_void, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_void,
                               P(ctypes.c_uint64), c_int]),
     'op2hip_function_attrs': (c_int, [c_void, P(c_int), P(c_int), P(c_int)]),

@@ -277,44 +277,6 @@ int op2hip_store_small(void* dptr, const void* src, size_t bytes, void* stream) 
   return OP2HIP_OK;
 }
 
-namespace {
-// Lane i copies byte i of a small device value into pinned host memory the
-// host reads after the stream has drained.
-__global__ void __launch_bounds__(kSmallBytes) load_small_bytes(const unsigned char* __restrict__ src,
-                                                                 unsigned char* __restrict__ dst,
-                                                                 int n) {
-  const int i = static_cast<int>(threadIdx.x);
-  if (i < n) dst[i] = src[i];
-}
-// the pinned staging bytes of op2hip_load_small (one host thread)
-unsigned char* g_small_host = nullptr;
-}  // namespace
-
-// Read up to 256 bytes at `dptr` into host memory `dst` after the work
-// already in `stream` (a Global's value): a kernel writes them into pinned
-// host memory, then the stream is drained.  Waits for the stream.
-int op2hip_load_small(void* dst, const void* dptr, size_t bytes, void* stream) {
-  OP2HIP_REQUIRE(dst && dptr, "null argument to op2hip_load_small");
-  OP2HIP_REQUIRE(bytes <= static_cast<size_t>(kSmallBytes),
-                 "op2hip_load_small takes at most %d bytes (got %zu)", kSmallBytes, bytes);
-  if (bytes == 0) return OP2HIP_OK;
-  if (!g_small_host) {
-    void* p = nullptr;
-    OP2HIP_CHECK(hipHostMalloc(&p, kSmallBytes, hipHostMallocMapped | hipHostMallocCoherent));
-    g_small_host = static_cast<unsigned char*>(p);
-  }
-  void* dev = nullptr;
-  OP2HIP_CHECK(hipHostGetDevicePointer(&dev, g_small_host, 0));
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(load_small_bytes, dim3(1), dim3(kSmallBytes), 0, s,
-                     static_cast<const unsigned char*>(dptr), static_cast<unsigned char*>(dev),
-                     static_cast<int>(bytes));
-  OP2HIP_CHECK(hipGetLastError());
-  OP2HIP_CHECK(hipStreamSynchronize(s));
-  std::memcpy(dst, g_small_host, bytes);
-  return OP2HIP_OK;
-}
-
 // Compile, load and look up in one call: `src` through hiprtc (with the
 // code-object cache), its module loaded, `entry` resolved.  The module is
 // the caller's (op2hip_module_destroy).

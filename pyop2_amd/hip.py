@@ -253,15 +253,6 @@ class Global(_DeviceData, base.Global):
         p = _pending[0]
         if p is not None and p.glob is self:
             flush()
-        a = self._data
-        if (getattr(self, '_state', HOST) == DEVICE and a.nbytes <= 256 and
-                a.flags['C_CONTIGUOUS']):
-            # a small value comes back through pinned memory a kernel writes
-            nat.check(nat.lib.op2hip_load_small(ctypes.c_void_p(a.ctypes.data),
-                                                ctypes.c_void_p(_ptr(self._dev)), a.nbytes,
-                                                ctypes.c_void_p(stream())), 'Global read')
-            self._state = BOTH
-            return
         _DeviceData._sync_to_host(self)
 
     def _upload(self):
@@ -1175,6 +1166,8 @@ def get_rows_plan(iterset, n, args, bs, lds_of, lds_limit, stage_maps=()):
 
 _owner_cache = {}
 OWNER_LDS_LIMIT = 64 * 1024
+# LDS kept free of dynamic allocations for the kernels' static arrays
+STATIC_LDS_MARGIN = 1024
 # MI355X (gfx950) compute unit: 160 KB of LDS, 4 SIMDs of at most 8 waves
 LDS_PER_CU = 160 * 1024
 WAVES_PER_CU = 32
@@ -1303,7 +1296,10 @@ def get_owner_plan(iterset, n, args, slot_bytes, k=None, block=256, lds_of=None,
         def build(kk):
             return OwnerPlan(n, target.total_size, vals, kk, order)
 
-    limit = cfg.get('hip_owner_lds') or min(OWNER_LDS_LIMIT, _state.get('lds_per_block') or 65536)
+    # the kernels' static LDS (the deferred combine pass's 128 bytes, Global
+    # reduction scratch) comes on top of the dynamic bytes sized here
+    limit = cfg.get('hip_owner_lds') or (min(OWNER_LDS_LIMIT, _state.get('lds_per_block') or 65536)
+                                         - STATIC_LDS_MARGIN)
     if lds_limit:
         limit = min(limit, lds_limit)
     if k_cfg:
@@ -1452,7 +1448,7 @@ class _LaunchEntry(object):
         lds = 0
         if mode == 'rows':
             limit = min(int(cfg.get('hip_mat_rows_lds') or ROWS_LDS_LIMIT),
-                        _state.get('lds_per_block') or 65536)
+                        (_state.get('lds_per_block') or 65536) - STATIC_LDS_MARGIN)
             smaps = [args[m].map for m in src.rows_maps()] if src.rows_stage else []
             plan = get_rows_plan(iterset, n, args, bs, src.rows_lds_bytes, limit, smaps)
             if plan is None:
