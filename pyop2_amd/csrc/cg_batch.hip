@@ -106,10 +106,33 @@ __device__ inline void chunk_rows(const Sys& s, int c, int& lo, int& hi) {
   if (hi > s.row0 + s.n) hi = s.row0 + s.n;
 }
 
+// The sums over a workgroup of NB threads of two values, each in
+// block_sum's order (valid in thread 0), with one barrier: the last use of
+// sh (2 * NB / 64 doubles) in the kernel.
+template <int NB>
+__device__ inline void block_sum_pair(double a, double b, double* sh, double& sa, double& sb) {
+  a = op2hip::wave_sum(a);
+  b = op2hip::wave_sum(b);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[w] = a;
+    sh[NB / 64 + w] = b;
+  }
+  __syncthreads();
+  double ta = 0.0, tb = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NB / 64; ++k) {
+      ta += sh[k];
+      tb += sh[NB / 64 + k];
+    }
+  sa = ta;
+  sb = tb;
+}
+
 // NEW systems: Jacobi diagonal, x = 0, r = b, z = D^-1 r, partials z.z, z.r
 __global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunks,
                                                  double* part) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[2 * kBlock / 64];
   const ChunkRef cr = chunks[blockIdx.x];
   Sys& s = sys[cr.sys];
   if (s.state != SYS_NEW) return;
@@ -141,62 +164,85 @@ __global__ void __launch_bounds__(kBlock) k_init(Sys* sys, const ChunkRef* chunk
     zz += zi * zi;
     zr += zi * ri;
   }
-  double t = block_sum<kBlock>(zz, sh);
-  double u = block_sum<kBlock>(zr, sh);
+  double t, u;
+  block_sum_pair<kBlock>(zz, zr, sh, t, u);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = t;
     part[2 * blockIdx.x + 1] = u;
   }
 }
 
-__device__ inline void reduce_pair(const Sys& s, const double* part, double* sh, double& a,
-                                   double& b) {
+// One thread's sums of a system's chunk partials (part[2 (c0 + c)] and
+// part[2 (c0 + c) + 1], chunks c = tid, tid + kRBlock, ... in that order),
+// with four chunks' loads in flight at once: a chunk past the end loads the
+// last chunk (c < nch, so nch >= 1) and adds +0.0 instead, which leaves a
+// sum that starts at +0.0 unchanged bit for bit (such a sum is never -0.0).
+__device__ inline void thread_partials(const double* part, int c0, int nch, double& a,
+                                       double& b) {
+  const OP2HIP_GLOBAL zp_pair* __restrict__ pp =
+      gmem(reinterpret_cast<const zp_pair*>(part)) + c0;
   double va = 0.0, vb = 0.0;
-  for (int c = threadIdx.x; c < s.nchunks; c += kRBlock) {
-    va += part[2 * (s.chunk0 + c)];
-    vb += part[2 * (s.chunk0 + c) + 1];
+  for (int c = threadIdx.x; c < nch; c += 4 * kRBlock) {
+    zp_pair q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int ck = c + k * kRBlock;
+      q[k] = pp[ck < nch ? ck : nch - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const bool in = c + k * kRBlock < nch;
+      va += in ? q[k].x : 0.0;
+      vb += in ? q[k].y : 0.0;
+    }
   }
-  a = block_sum<kRBlock>(va, sh);
-  b = block_sum<kRBlock>(vb, sh);
+  a = va;
+  b = vb;
 }
 
-__device__ inline void check(Sys& s) {
-  double dp = s.dp;
-  if (!(dp == dp) || isinf(dp)) {
-    s.reason = KSP_DIVERGED_NANORINF;
-    s.state = SYS_DONE;
-  } else if (dp <= s.ttol) {
-    s.reason = dp < s.atol ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
-    s.state = SYS_DONE;
-  } else if (dp >= s.dtol * s.rnorm0) {
-    s.reason = KSP_DIVERGED_DTOL;
-    s.state = SYS_DONE;
-  }
+// KSPConvergedDefault's test of dp: 0 to go on, else the reason
+__device__ inline int test_dp(double dp, double ttol, double atol, double dtol, double rnorm0) {
+  if (!(dp == dp) || isinf(dp)) return KSP_DIVERGED_NANORINF;
+  if (dp <= ttol) return dp < atol ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
+  if (dp >= dtol * rnorm0) return KSP_DIVERGED_DTOL;
+  return 0;
 }
+
+// The per-system kernels (k_start, k_alpha, k_finish) load every field of
+// their system they read at entry, in one round trip (an empty asm
+// statement taking the values keeps the compiler from sinking their loads
+// past the state test; it emits no instruction): thread 0 of the system's
+// workgroup is the only writer of the descriptor while they run, and it
+// writes after the loads.
+#define OP2HIP_LOADED(...) asm volatile("" ::__VA_ARGS__)
 
 // one workgroup per system
 __global__ void __launch_bounds__(kRBlock) k_start(Sys* sys, const double* part,
                                                   double* reshist, int hist_stride) {
-  __shared__ double sh[kRBlock / 64];
+  __shared__ double sh[2 * kRBlock / 64];
   Sys& s = sys[blockIdx.x];
-  if (s.state != SYS_NEW) return;
+  const int state = s.state, c0 = s.chunk0, nch = s.nchunks, maxit = s.maxit;
+  const double rtol = s.rtol, atol = s.atol, dtol = s.dtol;
+  OP2HIP_LOADED("s"(c0), "s"(nch), "s"(maxit), "s"(rtol), "s"(atol), "s"(dtol));
+  if (state != SYS_NEW) return;
+  double va, vb;
+  thread_partials(part, c0, nch, va, vb);
   double zz, zr;
-  reduce_pair(s, part, sh, zz, zr);
+  block_sum_pair<kRBlock>(va, vb, sh, zz, zr);
   if (threadIdx.x == 0) {
-    s.dp = sqrt(zz);
-    s.rnorm0 = s.dp;
-    s.ttol = fmax(s.rtol * s.rnorm0, s.atol);
+    const double dp = sqrt(zz);
+    const double ttol = fmax(rtol * dp, atol);
+    s.dp = dp;
+    s.rnorm0 = dp;
+    s.ttol = ttol;
     s.its = 0;
-    s.reason = 0;
     s.dpi = 0.0;
-    s.state = SYS_RUNNING;
-    if (reshist) reshist[(long long)blockIdx.x * hist_stride] = s.dp;
-    check(s);
+    if (reshist) reshist[(long long)blockIdx.x * hist_stride] = dp;
+    int reason = test_dp(dp, ttol, atol, dtol, dp);
     s.beta = zr;
-    if (s.state == SYS_RUNNING && s.maxit <= 0) {
-      s.reason = KSP_DIVERGED_ITS;
-      s.state = SYS_DONE;
-    }
+    if (reason == 0 && maxit <= 0) reason = KSP_DIVERGED_ITS;
+    s.reason = reason;
+    s.state = reason != 0 ? SYS_DONE : SYS_RUNNING;
   }
 }
 
@@ -280,32 +326,36 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
 }
 
 __global__ void __launch_bounds__(kRBlock) k_alpha(Sys* sys, const double* part) {
-  __shared__ double sh[kRBlock / 64];
+  __shared__ double sh[2 * kRBlock / 64];
   Sys& s = sys[blockIdx.x];
-  if (s.state != SYS_RUNNING) return;
-  double v = 0.0;
-  for (int c = threadIdx.x; c < s.nchunks; c += kRBlock) v += part[2 * (s.chunk0 + c)];
-  double pw = block_sum<kRBlock>(v, sh);
+  const int state = s.state, c0 = s.chunk0, nch = s.nchunks, its = s.its + 1;
+  const double beta = s.beta, dpiold = s.dpi;
+  OP2HIP_LOADED("s"(c0), "s"(nch), "s"(its), "s"(beta), "s"(dpiold));
+  if (state != SYS_RUNNING) return;
+  double va, vb;
+  thread_partials(part, c0, nch, va, vb);
+  double pw, unused;
+  block_sum_pair<kRBlock>(va, 0.0, sh, pw, unused);
   if (threadIdx.x == 0) {
-    s.its += 1;
-    if (s.beta == 0.0) {
+    s.its = its;
+    if (beta == 0.0) {
       s.reason = KSP_CONVERGED_ATOL;
       s.state = SYS_DONE;
       return;
     }
-    s.dpiold = s.dpi;
+    s.dpiold = dpiold;
     s.dpi = pw;
-    if (pw == 0.0 || (s.its > 1 && s.dpi * s.dpiold <= 0.0)) {
+    if (pw == 0.0 || (its > 1 && pw * dpiold <= 0.0)) {
       s.reason = KSP_DIVERGED_INDEFINITE_MAT;
       s.state = SYS_DONE;
       return;
     }
-    s.alpha = s.beta / pw;
+    s.alpha = beta / pw;
   }
 }
 
 __global__ void __launch_bounds__(kBlock) k_axpy(Sys* sys, const ChunkRef* chunks, double* part) {
-  __shared__ double sh[kBlock / 64];
+  __shared__ double sh[2 * kBlock / 64];
   const ChunkRef cr = chunks[blockIdx.x];
   Sys& s = sys[cr.sys];
   if (s.state != SYS_RUNNING) return;
@@ -332,8 +382,8 @@ __global__ void __launch_bounds__(kBlock) k_axpy(Sys* sys, const ChunkRef* chunk
     zz += zi * zi;
     zr += zi * ri;
   }
-  double t = block_sum<kBlock>(zz, sh);
-  double u = block_sum<kBlock>(zr, sh);
+  double t, u;
+  block_sum_pair<kBlock>(zz, zr, sh, t, u);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = t;
     part[2 * blockIdx.x + 1] = u;
@@ -342,22 +392,28 @@ __global__ void __launch_bounds__(kBlock) k_axpy(Sys* sys, const ChunkRef* chunk
 
 __global__ void __launch_bounds__(kRBlock) k_finish(Sys* sys, const double* part, double* reshist,
                                                    int hist_stride) {
-  __shared__ double sh[kRBlock / 64];
+  __shared__ double sh[2 * kRBlock / 64];
   Sys& s = sys[blockIdx.x];
-  if (s.state != SYS_RUNNING) return;
+  const int state = s.state, c0 = s.chunk0, nch = s.nchunks, its = s.its, maxit = s.maxit;
+  const double ttol = s.ttol, atol = s.atol, dtol = s.dtol, rnorm0 = s.rnorm0, beta = s.beta;
+  OP2HIP_LOADED("s"(c0), "s"(nch), "s"(its), "s"(maxit), "s"(ttol), "s"(atol), "s"(dtol),
+                "s"(rnorm0), "s"(beta));
+  if (state != SYS_RUNNING) return;
+  double va, vb;
+  thread_partials(part, c0, nch, va, vb);
   double zz, zr;
-  reduce_pair(s, part, sh, zz, zr);
+  block_sum_pair<kRBlock>(va, vb, sh, zz, zr);
   if (threadIdx.x == 0) {
-    s.dp = sqrt(zz);
-    if (reshist && s.its < hist_stride)
-      reshist[(long long)blockIdx.x * hist_stride + s.its] = s.dp;
-    check(s);
-    if (s.state == SYS_RUNNING && s.its >= s.maxit) {
-      s.reason = KSP_DIVERGED_ITS;
+    const double dp = sqrt(zz);
+    s.dp = dp;
+    if (reshist && its < hist_stride) reshist[(long long)blockIdx.x * hist_stride + its] = dp;
+    int reason = test_dp(dp, ttol, atol, dtol, rnorm0);
+    if (reason == 0 && its >= maxit) reason = KSP_DIVERGED_ITS;
+    if (reason != 0) {
+      s.reason = reason;
       s.state = SYS_DONE;
-    }
-    if (s.state == SYS_RUNNING) {
-      s.betaold = s.beta;
+    } else {
+      s.betaold = beta;
       s.beta = zr;
     }
   }
