@@ -315,16 +315,31 @@ __global__ void __launch_bounds__(kBlock) k_g_dots(Prob p) {
 }
 
 // Gram-Schmidt coefficients h_k = w . v_k into column j of H; lhh = -h.
-// One wavefront per column, all columns at once.
+// One wavefront per column, all columns at once.  A lane adds its partials
+// b = lane, lane + 64, ... in that order, loading 16 of them before the
+// first wait: a partial past the end loads the last one (b < nb, so nb >= 1)
+// and adds +0.0 instead, which leaves a sum that starts at +0.0 unchanged
+// bit for bit (such a sum is never -0.0).
 __global__ void __launch_bounds__(kRBlock) k_g_coef(Prob p) {
+  constexpr int kBatch = 16;
   const State& s = *p.st;
   if (s.phase != PH_STEP) return;
   const int j = s.it;
   const int lane = threadIdx.x & 63;
+  const int nb = p.nb;
   for (int k = threadIdx.x >> 6; k <= j; k += kRBlock / 64) {
     double v = 0.0;
     const double* __restrict__ pk = p.part + (size_t)k * p.ps;
-    for (int b = lane; b < p.nb; b += 64) v += pk[b];
+    for (int b0 = lane; b0 < nb; b0 += kBatch * 64) {
+      double t[kBatch];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        const int b = b0 + q * 64;
+        t[q] = pk[b < nb ? b : nb - 1];
+      }
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) v += b0 + q * 64 < nb ? t[q] : 0.0;
+    }
     v = op2hip::wave_sum(v);
     if (lane == 0) {
       p.lhh[k] = -v;
