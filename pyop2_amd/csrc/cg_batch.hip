@@ -209,12 +209,10 @@ __device__ inline int test_dp(double dp, double ttol, double atol, double dtol, 
 }
 
 // The per-system kernels (k_start, k_alpha, k_finish) load every field of
-// their system they read at entry, in one round trip (an empty asm
-// statement taking the values keeps the compiler from sinking their loads
-// past the state test; it emits no instruction): thread 0 of the system's
-// workgroup is the only writer of the descriptor while they run, and it
-// writes after the loads.
-#define OP2HIP_LOADED(...) asm volatile("" ::__VA_ARGS__)
+// their system they read at entry, in one round trip (OP2HIP_LOADED keeps
+// the loads ahead of the state test): thread 0 of the system's workgroup is
+// the only writer of the descriptor while they run, and it writes after
+// the loads.
 
 // one workgroup per system
 __global__ void __launch_bounds__(kRBlock) k_start(Sys* sys, const double* part,

@@ -96,15 +96,22 @@ struct Prob {
   double *R, *RP, *Vb, *T, *S, *P;
 };
 
+// KSPConvergedDefault's test of rn once rnorm0 and ttol are set: 0 to go
+// on, else the reason
+__device__ inline int ksp_test_rn(double rn, double ttol, double atol, double dtol,
+                                  double rnorm0) {
+  if (!(rn == rn) || isinf(rn)) return KSP_DIVERGED_NANORINF;
+  if (rn <= ttol) return rn < atol ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
+  if (rn >= dtol * rnorm0) return KSP_DIVERGED_DTOL;
+  return 0;
+}
+
 __device__ inline int ksp_test(State& s, double rn) {
   if (s.its == 0) {
     s.rnorm0 = rn;
     s.ttol = fmax(s.rtol * rn, s.atol);
   }
-  if (!(rn == rn) || isinf(rn)) return KSP_DIVERGED_NANORINF;
-  if (rn <= s.ttol) return rn < s.atol ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
-  if (rn >= s.dtol * s.rnorm0) return KSP_DIVERGED_DTOL;
-  return 0;
+  return ksp_test_rn(rn, s.ttol, s.atol, s.dtol, s.rnorm0);
 }
 
 __device__ inline void log_res(const Prob& p, int k, double v) {
@@ -113,15 +120,33 @@ __device__ inline void log_res(const Prob& p, int k, double v) {
 
 // sums of partials k0 .. k0+K-1 over the nparts workgroups of a row
 // kernel, in one pass (one workgroup of kRBlock; sh holds K * kRBlock / 64
-// doubles); the results are valid in thread 0
+// doubles); the results are valid in thread 0.  A thread adds its
+// partials b = tid, tid + kRBlock, ... in that order, loading four of each
+// quantity before the first wait: a partial past the end loads the last
+// one (b < nparts, so nparts >= 1) and adds +0.0 instead, which leaves a
+// sum that starts at +0.0 unchanged bit for bit (such a sum is never -0.0).
 template <int K>
 __device__ inline void reduce_parts(const Prob& p, int nparts, int k0, double* out, double* sh) {
+  constexpr int kBatch = 4;
   double v[K];
 #pragma unroll
   for (int q = 0; q < K; ++q) v[q] = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += kRBlock) {
+  const double* __restrict__ base = p.part + (size_t)k0 * p.ps;
+  for (int b0 = threadIdx.x; b0 < nparts; b0 += kBatch * kRBlock) {
+    double t[kBatch][K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) v[q] += p.part[(size_t)(k0 + q) * p.ps + b];
+    for (int r = 0; r < kBatch; ++r) {
+      const int b = b0 + r * kRBlock;
+      const int bc = b < nparts ? b : nparts - 1;
+#pragma unroll
+      for (int q = 0; q < K; ++q) t[r][q] = base[(size_t)q * p.ps + bc];
+    }
+#pragma unroll
+    for (int r = 0; r < kBatch; ++r) {
+      const bool in = b0 + r * kRBlock < nparts;
+#pragma unroll
+      for (int q = 0; q < K; ++q) v[q] += in ? t[r][q] : 0.0;
+    }
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -591,10 +616,16 @@ __global__ void __launch_bounds__(kBlock) k_b_spmv_p(Prob p) {
   store_part(p, 0, acc, sh);
 }
 
+// (k_b_alpha and k_b_finish load the State fields they read at entry, in
+// one round trip with the phase; thread 0 is the only writer of the State
+// while they run, and it writes after the loads.)
 __global__ void __launch_bounds__(kRBlock) k_b_alpha(Prob p) {
   __shared__ double sh[kRBlock / 64];
   State& s = *p.st;
-  if (s.phase != PH_STEP) return;
+  const int phase = s.phase;
+  const double rho = s.rho;
+  OP2HIP_LOADED("s"(rho));
+  if (phase != PH_STEP) return;
   const double d1 = reduce_part(p, p.nbs, 0, sh);
   if (threadIdx.x != 0) return;
   if (d1 == 0.0) {
@@ -602,7 +633,7 @@ __global__ void __launch_bounds__(kRBlock) k_b_alpha(Prob p) {
     s.phase = PH_DONE;
     return;
   }
-  s.alpha = s.rho / d1;
+  s.alpha = rho / d1;
 }
 
 // s = r - alpha v (formed where gathered, stored per row), t = D^-1 A s,
@@ -687,30 +718,37 @@ __global__ void __launch_bounds__(kBlock) k_b_update(Prob p) {
 __global__ void __launch_bounds__(kRBlock) k_b_finish(Prob p) {
   __shared__ double sh[2 * kRBlock / 64];
   State& s = *p.st;
-  if (s.phase == PH_FINAL_X) {
+  const int phase = s.phase, its = s.its + 1;
+  const double rho = s.rho, omega = s.omega, alpha = s.alpha;
+  const double ttol = s.ttol, atol = s.atol, dtol = s.dtol, rnorm0 = s.rnorm0;
+  OP2HIP_LOADED("s"(its), "s"(rho), "s"(omega), "s"(alpha), "s"(ttol), "s"(atol), "s"(dtol),
+                "s"(rnorm0));
+  if (phase == PH_FINAL_X) {
     if (threadIdx.x == 0) s.phase = PH_DONE;
     return;
   }
-  if (s.phase != PH_STEP) return;
+  if (phase != PH_STEP) return;
   double d[2];
   reduce_parts<2>(p, p.nb, 0, d, sh);
   if (threadIdx.x != 0) return;
   const double rr = d[0], rrp = d[1];
   const double dp = sqrt(rr);
-  s.rhoold = s.rho;
-  s.omegaold = s.omega;
-  s.its += 1;
+  s.rhoold = rho;
+  s.omegaold = omega;
+  s.its = its;
   s.res = dp;
-  log_res(p, s.its, dp);
-  s.reason = ksp_test(s, dp);
-  if (!s.reason && s.rho == 0.0) s.reason = KSP_DIVERGED_BREAKDOWN;
-  if (!s.reason && s.its >= p.maxit) s.reason = KSP_DIVERGED_ITS;
-  if (s.reason) {
+  log_res(p, its, dp);
+  // its >= 1: rnorm0 and ttol were set by the first test
+  int reason = ksp_test_rn(dp, ttol, atol, dtol, rnorm0);
+  if (!reason && rho == 0.0) reason = KSP_DIVERGED_BREAKDOWN;
+  if (!reason && its >= p.maxit) reason = KSP_DIVERGED_ITS;
+  s.reason = reason;
+  if (reason) {
     s.phase = PH_DONE;
     return;
   }
   s.rho = rrp;
-  s.beta = (s.rho / s.rhoold) * (s.alpha / s.omegaold);
+  s.beta = (rrp / rho) * (alpha / omega);
 }
 
 // --- host -----------------------------------------------------------------------------

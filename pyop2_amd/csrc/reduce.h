@@ -3,6 +3,12 @@
 
 #include <hip/hip_runtime.h>
 
+// Takes values as operands of an empty asm statement (no instruction), so
+// the compiler issues their loads before it: a kernel loads the fields it
+// reads from a descriptor in one round trip ahead of a test that would
+// otherwise have the loads sunk past it.
+#define OP2HIP_LOADED(...) asm volatile("" ::__VA_ARGS__)
+
 namespace op2hip {
 
 // sum over the 64 lanes of a wavefront (every lane gets the total)
