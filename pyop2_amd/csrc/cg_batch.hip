@@ -100,10 +100,14 @@ struct ChunkRef {
   int sys, chunk;
 };
 
-__device__ inline void chunk_rows(const Sys& s, int c, int& lo, int& hi) {
-  lo = s.row0 + c * kChunk;
+__device__ inline void chunk_rows(int row0, int n, int c, int& lo, int& hi) {
+  lo = row0 + c * kChunk;
   hi = lo + kChunk;
-  if (hi > s.row0 + s.n) hi = s.row0 + s.n;
+  if (hi > row0 + n) hi = row0 + n;
+}
+
+__device__ inline void chunk_rows(const Sys& s, int c, int& lo, int& hi) {
+  chunk_rows(s.row0, s.n, c, lo, hi);
 }
 
 // The sums over a workgroup of NB threads of two values, each in
@@ -250,15 +254,26 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
                                                  double* part) {
   __shared__ double sh[kBlock / 64];
   const ChunkRef cr = chunks[blockIdx.x];
-  Sys& s = sys[cr.sys];
-  if (s.state != SYS_RUNNING) return;
+  const Sys& s = sys[cr.sys];
+  // every descriptor field read here, loaded with the state in one round
+  // trip (OP2HIP_LOADED keeps the loads ahead of the state test)
+  const int state = s.state, row0 = s.row0, nrows = s.n, its = s.its;
+  const double beta = s.beta, betaold = s.betaold;
+  const int* rp = s.rowptr;
+  const int* ci = s.colidx;
+  const double* va = s.vals;
+  const double* zpp = s.zp;
+  double* wp = s.w;
+  OP2HIP_LOADED("s"(cr.chunk), "s"(row0), "s"(nrows), "s"(its), "s"(beta), "s"(betaold), "s"(rp),
+                "s"(ci), "s"(va), "s"(zpp), "s"(wp));
+  if (state != SYS_RUNNING) return;
   int lo, hi;
-  chunk_rows(s, cr.chunk, lo, hi);
-  const OP2HIP_GLOBAL int* __restrict__ rowptr = gmem(s.rowptr);
-  const OP2HIP_GLOBAL int* __restrict__ colidx = gmem(s.colidx);
-  const OP2HIP_GLOBAL double* __restrict__ vals = gmem(s.vals);
-  const OP2HIP_GLOBAL zp_pair* __restrict__ zp = gmem(reinterpret_cast<const zp_pair*>(s.zp));
-  OP2HIP_GLOBAL double* __restrict__ w = gmem(s.w);
+  chunk_rows(row0, nrows, cr.chunk, lo, hi);
+  const OP2HIP_GLOBAL int* __restrict__ rowptr = gmem(rp);
+  const OP2HIP_GLOBAL int* __restrict__ colidx = gmem(ci);
+  const OP2HIP_GLOBAL double* __restrict__ vals = gmem(va);
+  const OP2HIP_GLOBAL zp_pair* __restrict__ zp = gmem(reinterpret_cast<const zp_pair*>(zpp));
+  OP2HIP_GLOBAL double* __restrict__ w = gmem(wp);
   // lanes per row from the chunk's mean row length (about two entries per
   // lane: 4 lanes for 2-D P1 rows, 8 for 3-D, 16 for 3-D vector rows); a
   // property of the chunk alone, so a system sums alike in any batch
@@ -271,8 +286,8 @@ __global__ void __launch_bounds__(kBlock) k_spmv(Sys* sys, const ChunkRef* chunk
   // the new direction p' = z (first iteration) or z + (beta/betaold) p,
   // formed where it is read: the gathered columns here (one 16-byte load
   // of z and p per entry), each row again in k_axpy, which stores it
-  const bool first = s.its == 0;
-  const double bb = first ? 0.0 : s.beta / s.betaold;
+  const bool first = its == 0;
+  const double bb = first ? 0.0 : beta / betaold;
   double acc = 0.0;
   // Two rows per lane group (i0 and i0 + stride) and two entries of each
   // per lane are loaded together, so four gathers are in flight per lane.
@@ -355,20 +370,29 @@ __global__ void __launch_bounds__(kRBlock) k_alpha(Sys* sys, const double* part)
 __global__ void __launch_bounds__(kBlock) k_axpy(Sys* sys, const ChunkRef* chunks, double* part) {
   __shared__ double sh[2 * kBlock / 64];
   const ChunkRef cr = chunks[blockIdx.x];
-  Sys& s = sys[cr.sys];
-  if (s.state != SYS_RUNNING) return;
+  const Sys& s = sys[cr.sys];
+  // every descriptor field read here, loaded with the state in one round trip
+  const int state = s.state, row0 = s.row0, nrows = s.n, its = s.its;
+  const double a = s.alpha, beta = s.beta, betaold = s.betaold;
+  double* xp = s.x;
+  double* rp = s.r;
+  const double* wp = s.w;
+  const double* dp = s.inv_diag;
+  double* zpp = s.zp;
+  OP2HIP_LOADED("s"(cr.chunk), "s"(row0), "s"(nrows), "s"(its), "s"(a), "s"(beta), "s"(betaold),
+                "s"(xp), "s"(rp), "s"(wp), "s"(dp), "s"(zpp));
+  if (state != SYS_RUNNING) return;
   int lo, hi;
-  chunk_rows(s, cr.chunk, lo, hi);
+  chunk_rows(row0, nrows, cr.chunk, lo, hi);
   double zz = 0.0, zr = 0.0;
-  const double a = s.alpha;
-  OP2HIP_GLOBAL double* __restrict__ x = gmem(s.x);
-  OP2HIP_GLOBAL double* __restrict__ r = gmem(s.r);
-  const OP2HIP_GLOBAL double* __restrict__ w = gmem(s.w);
-  const OP2HIP_GLOBAL double* __restrict__ inv_diag = gmem(s.inv_diag);
-  OP2HIP_GLOBAL zp_pair* __restrict__ zp = gmem(reinterpret_cast<zp_pair*>(s.zp));
+  OP2HIP_GLOBAL double* __restrict__ x = gmem(xp);
+  OP2HIP_GLOBAL double* __restrict__ r = gmem(rp);
+  const OP2HIP_GLOBAL double* __restrict__ w = gmem(wp);
+  const OP2HIP_GLOBAL double* __restrict__ inv_diag = gmem(dp);
+  OP2HIP_GLOBAL zp_pair* __restrict__ zp = gmem(reinterpret_cast<zp_pair*>(zpp));
   // k_alpha has counted this iteration: its == 1 on the first
-  const bool first = s.its == 1;
-  const double bb = first ? 0.0 : s.beta / s.betaold;
+  const bool first = its == 1;
+  const double bb = first ? 0.0 : beta / betaold;
   for (int i = lo + (int)threadIdx.x; i < hi; i += kBlock) {
     const zp_pair q = zp[i];
     const double pi = first ? q.x : q.x + bb * q.y;
